@@ -1,0 +1,185 @@
+/*
+ * pgt_ik.h -- C ABI of the MI355X Irving-Kirkwood stress-field engine
+ * (libpgt_ik.so).
+ *
+ * Plain C types only: host pointers unless a name says `d_` (device pointer
+ * from hipMalloc or any HIP allocator on the engine's device).  Every entry
+ * point returns PGT_OK (0) or a negative error code; pgt_last_error() returns
+ * the calling thread's message for the last failure.  No C++ exception crosses
+ * this boundary.  One engine is used by one host thread at a time.
+ *
+ * The reference (mandadapu-group/pyglasstools) binds these operations through
+ * pybind11 classes; each declaration names the reference interface it
+ * replaces (paths relative to the reference checkout).
+ */
+#ifndef PGT_IK_H
+#define PGT_IK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGT_OK 0
+#define PGT_ERR_ARG (-1)       /* invalid argument (reference: std::invalid_argument -> ValueError) */
+#define PGT_ERR_HIP (-2)       /* HIP runtime error, including "no GPU" */
+#define PGT_ERR_STATE (-3)     /* call order: e.g. compute before set_particles */
+#define PGT_ERR_RANGE (-4)     /* search radius >= half a periodic box length */
+#define PGT_ERR_UNSUPPORTED (-5)
+
+/* Pair potentials of the polydisperse m-n + even-polynomial family.
+ * Reference: pyglasstools/potential/module-potential.cpp:17-19,26-30 */
+enum pgt_potential_kind {
+    PGT_POT_POLY12 = 0,  /* Polydisperse12.h:38-54, params {v0, eps, c0, c1, c2} */
+    PGT_POT_POLY18 = 1,  /* Polydisperse18.h:37-53, params {v0, eps, c0, c1, c2} */
+    PGT_POT_POLY10 = 2,  /* Polydisperse10.h:37-52, params {v0, eps, c0, c1, c2, c3} */
+    PGT_POT_POLY106 = 3, /* Polydisperse106.h:37-53, params {v0, eps, c0, c1, c2} */
+    PGT_POT_POLYLJ = 4   /* PolydisperseLJ.h:37-53, params {v0, eps, c0, c1, c2} */
+};
+
+/* Coarse-graining kernels.  Reference: irvingkirkwood/cgfunc/module-cgfunc.cpp:8-16 */
+enum pgt_cg_kind {
+    PGT_CG_OCTIC = 0,    /* CGFuncOctic.h:16-29 */
+    PGT_CG_RECT = 1,     /* CGFuncRect.h:17-28 */
+    PGT_CG_MOLLIFIER = 2 /* CGFuncMollifier.h:11-31 */
+};
+
+/* Bond line-integral rule.  Reference: CoarseGrainFunction.h:56-134 */
+enum pgt_quad_mode {
+    PGT_QUAD_FIXED = 0,   /* FixedPointCGFunc: n-point Gauss-Legendre on [0,1] */
+    PGT_QUAD_ADAPTIVE = 1 /* AdaptiveCGFunc: QAG with 15-point Gauss-Kronrod */
+};
+
+/* Arithmetic of the field kernels (the reference has FP64 only). */
+enum pgt_precision {
+    PGT_PREC_FP64 = 0, /* the drop-in: FP64 everywhere */
+    PGT_PREC_FP32 = 1  /* FP32 pair forces and kernel weights; FP64 minimum image
+                          and FP64 per-grid-point accumulation */
+};
+
+/* Field selection bits for pgt_ik_compute*.  Reference field classes:
+ * irvingkirkwood/module-irvingkirkwood.cpp:8-13 */
+enum pgt_field_bits {
+    PGT_FIELD_VIRIAL = 1,  /* VirialStressField2D/3D        (ListofCommonObservables.h:43-84) */
+    PGT_FIELD_KINETIC = 2, /* KineticStressField2D/3D       (ListofCommonObservables.h:125-149) */
+    PGT_FIELD_ELASTIC = 4  /* ElasticVirialStressField2D/3D (ListofCommonObservables.h:86-123) */
+};
+
+/* Everything IrvingKirkwood's constructor receives, flattened:
+ * pair potential (PairPotential.h:30-33), coarse-graining function
+ * (CoarseGrainFunction.h:64-67,102-105), and Calculator::max_rcut
+ * (Calculator.h:37-46), which the caller computes from the diameters of the
+ * system the calculator is built with and which stays fixed afterwards. */
+typedef struct pgt_ik_params {
+    int dim;                  /* 2 or 3: size of the output tensors */
+    int potential_kind;       /* enum pgt_potential_kind */
+    double scaled_rcut;       /* PairPotential::scaled_rcut */
+    double pot_params[6];     /* PairPotential::params */
+    int n_pot_params;
+    int cg_kind;              /* enum pgt_cg_kind */
+    double cg_rcut;           /* CoarseGrainFunction::cg_rcut */
+    int quad_mode;            /* enum pgt_quad_mode */
+    int quad_order;           /* fixed rule: number of nodes (1..1024) */
+    int adapt_limit;          /* adaptive rule: GSL workspace limit (neval) */
+    double adapt_epsrel;
+    double adapt_epsabs;
+    int precision;            /* enum pgt_precision */
+    double max_rcut;          /* j search radius */
+} pgt_ik_params;
+
+typedef struct pgt_ik_stats {
+    int64_t n_particles;
+    int64_t n_grid;
+    int64_t n_ball;        /* (grid point, particle in ball) pairs, last compute */
+    int64_t n_contrib;     /* U: (grid point, particle, neighbour) with F != 0 */
+    int32_t bond_capacity; /* per-particle bond slots in use */
+    int32_t max_bonds;     /* largest per-particle bond count seen */
+    double gather_ms;      /* HIP-event time of the field gather kernel(s), last compute */
+    double total_ms;       /* HIP-event time of the whole last compute */
+} pgt_ik_stats;
+
+typedef struct pgt_ik_engine pgt_ik_engine;
+
+/* Thread-local message for the last failed call on this thread. */
+const char *pgt_last_error(void);
+
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int pgt_device_count(void);
+
+/* Create an engine on the current HIP device.
+ * Replaces: IrvingKirkwood(ParticleSystem, PairPotential, CoarseGrainFunction,
+ * Communicator), irvingkirkwood/IrvingKirkwood.h:12-18, 116. */
+int pgt_ik_create(const pgt_ik_params *params, pgt_ik_engine **out);
+void pgt_ik_destroy(pgt_ik_engine *e);
+
+/* Box lengths, lower bounds and periodic flags (z non-periodic in 2D).
+ * Replaces: SimBox(boxsize, origin, dim) + setBounds, SimBox.cc:3-52. */
+int pgt_ik_set_box(pgt_ik_engine *e, const double boxsize[3], const double lowerbound[3],
+                   const int periodic[3]);
+
+/* Host particle data, N x 3 row-major positions/velocities/displacements.
+ * mass, vel and disp may be NULL (mass 1, zero velocity/displacement).
+ * Replaces: ParticleSystem(simbox, N, diameter, mass, position, velocity)
+ * (ParticleSystem.cc:3-30) and IrvingKirkwood::setSystemData
+ * (Calculator.h:50-53); the neighbour index is built on the GPU at compute. */
+int pgt_ik_set_particles(pgt_ik_engine *e, int64_t n, const double *pos, const double *diam,
+                         const double *mass, const double *vel, const double *disp);
+
+/* Host grid points, G x 3 row-major.
+ * Replaces: the gridpoints argument of IrvingKirkwood::compute (IrvingKirkwood.h:21). */
+int pgt_ik_set_grid(pgt_ik_engine *e, int64_t g, const double *grid);
+
+/* Compute the selected fields for the current particles and grid and copy
+ * them to host arrays of G x D x D doubles (flat index a + D*b per grid
+ * point, the layout gettostring reads, CoarseGrainedField.h:160-183).
+ * Unselected outputs may be NULL.
+ * Replaces: IrvingKirkwood::compute, irvingkirkwood/IrvingKirkwood.h:74-110. */
+int pgt_ik_compute(pgt_ik_engine *e, unsigned fields, double *virial, double *kinetic,
+                   double *elastic);
+
+/* Batched, device-resident: nframes systems of n particles each, all on the
+ * device, over the grid given by pgt_ik_set_grid.  Frame f reads
+ * d_pos + f*3n, d_diam + f*n, d_mass + f*n (NULL: 1), d_vel + f*3n (NULL: 0),
+ * d_disp + f*3n (NULL: 0) and writes G*D*D doubles at d_out_* + f*G*D*D.
+ * No host synchronisation inside unless a bond list outgrows its capacity.
+ * Replaces: the per-frame loop of pyglasstools.analyze (__init__.py:119-122)
+ * around IrvingKirkwood::compute, for callers that keep frames on the GPU. */
+int pgt_ik_compute_frames(pgt_ik_engine *e, int nframes, int64_t n, const double *d_pos,
+                          const double *d_diam, const double *d_mass, const double *d_vel,
+                          const double *d_disp, unsigned fields, double *d_out_virial,
+                          double *d_out_kinetic, double *d_out_elastic);
+
+/* Run the engine's work on the given hipStream_t (NULL: the engine's own). */
+int pgt_ik_set_stream(pgt_ik_engine *e, void *hip_stream);
+
+/* Lanes per grid point (1,2,4,8,16) and frames per launch used by the gather
+ * kernel; 0 keeps the current choice.  Used by the autotuner. */
+int pgt_ik_set_tuning(pgt_ik_engine *e, int lanes_per_point, int frames_per_launch);
+
+int pgt_ik_get_stats(pgt_ik_engine *e, pgt_ik_stats *out);
+
+/* Batched evaluators on the GPU, host arrays in and out (n x 3 vectors).
+ * pgt_eval_pair_force replaces PairPotential::getPairForceDivR / getPairEnergy /
+ * getRcut (PairPotential.h:89-113); out_* may be NULL. */
+int pgt_eval_pair_force(int potential_kind, const double *pot_params, int n_pot_params,
+                        double scaled_rcut, int64_t n, const double *rij, const double *di,
+                        const double *dj, double *out_force_divr, double *out_energy,
+                        double *out_rcutsq);
+
+/* mode 0: Delta(x, ri)                 (CoarseGrainFunction.h:78-83, getDeltaFunc)
+ * mode 1: Delta(x, ri + s*rij)         (CoarseGrainFunction.h:42-45, getObjFunc)
+ * mode 2: bond integral over s in [0,1] (CoarseGrainFunction.h:85-88 fixed,
+ *                                        :124-127 adaptive, getBondFunc)
+ * s may be NULL unless mode == 1; rij may be NULL for mode 0. */
+int pgt_eval_cgfunc(int cg_kind, double cg_rcut, int quad_mode, int quad_order, int adapt_limit,
+                    double adapt_epsrel, double adapt_epsabs, int mode, int64_t n,
+                    const double *s, const double *x, const double *ri, const double *rij,
+                    double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PGT_IK_H */

@@ -1,0 +1,257 @@
+"""ctypes binding of the C ABI in include/pgt_ik.h (libpgt_ik.so).
+
+This is the boundary the tests and the benchmark call: plain pointers and
+sizes, host numpy arrays for pgt_ik_compute, device pointers (any HIP
+allocation, e.g. torch tensors' data_ptr()) for pgt_ik_compute_frames.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpgt_ik.so")
+
+PGT_OK = 0
+ERRORS = {-1: "PGT_ERR_ARG", -2: "PGT_ERR_HIP", -3: "PGT_ERR_STATE", -4: "PGT_ERR_RANGE", -5: "PGT_ERR_UNSUPPORTED"}
+POT = {"poly12": 0, "poly18": 1, "poly10": 2, "poly106": 3, "polylj": 4}
+CG = {"octic": 0, "rect": 1, "mollifier": 2}
+FIELD_VIRIAL, FIELD_KINETIC, FIELD_ELASTIC = 1, 2, 4
+PREC = {"fp64": 0, "fp32": 1}
+
+# every symbol include/pgt_ik.h declares
+EXPORTED = [
+    "pgt_last_error", "pgt_device_count", "pgt_ik_create", "pgt_ik_destroy", "pgt_ik_set_box",
+    "pgt_ik_set_particles", "pgt_ik_set_grid", "pgt_ik_compute", "pgt_ik_compute_frames", "pgt_ik_set_stream",
+    "pgt_ik_set_tuning", "pgt_ik_get_stats", "pgt_eval_pair_force", "pgt_eval_cgfunc",
+]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("potential_kind", ctypes.c_int),
+        ("scaled_rcut", ctypes.c_double),
+        ("pot_params", ctypes.c_double * 6),
+        ("n_pot_params", ctypes.c_int),
+        ("cg_kind", ctypes.c_int),
+        ("cg_rcut", ctypes.c_double),
+        ("quad_mode", ctypes.c_int),
+        ("quad_order", ctypes.c_int),
+        ("adapt_limit", ctypes.c_int),
+        ("adapt_epsrel", ctypes.c_double),
+        ("adapt_epsabs", ctypes.c_double),
+        ("precision", ctypes.c_int),
+        ("max_rcut", ctypes.c_double),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("n_particles", ctypes.c_int64),
+        ("n_grid", ctypes.c_int64),
+        ("n_ball", ctypes.c_int64),
+        ("n_contrib", ctypes.c_int64),
+        ("bond_capacity", ctypes.c_int32),
+        ("max_bonds", ctypes.c_int32),
+        ("gather_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("%s is not built; run `python -m pyglasstools_amd.csrc.build`" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, d, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
+        L.pgt_last_error.restype = ctypes.c_char_p
+        L.pgt_device_count.restype = i
+        L.pgt_ik_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(vp)]
+        L.pgt_ik_destroy.argtypes = [vp]
+        L.pgt_ik_destroy.restype = None
+        L.pgt_ik_set_box.argtypes = [vp, vp, vp, vp]
+        L.pgt_ik_set_particles.argtypes = [vp, i64, vp, vp, vp, vp, vp]
+        L.pgt_ik_set_grid.argtypes = [vp, i64, vp]
+        L.pgt_ik_compute.argtypes = [vp, ctypes.c_uint, vp, vp, vp]
+        L.pgt_ik_compute_frames.argtypes = [vp, i, i64, vp, vp, vp, vp, vp, ctypes.c_uint, vp, vp, vp]
+        L.pgt_ik_set_stream.argtypes = [vp, vp]
+        L.pgt_ik_set_tuning.argtypes = [vp, i, i]
+        L.pgt_ik_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+        L.pgt_eval_pair_force.argtypes = [i, vp, i, d, i64, vp, vp, vp, vp, vp, vp]
+        L.pgt_eval_cgfunc.argtypes = [i, d, i, i, i, d, d, i, i64, vp, vp, vp, vp, vp]
+        for name in EXPORTED:
+            if name not in ("pgt_last_error", "pgt_device_count", "pgt_ik_destroy"):
+                getattr(L, name).restype = i
+        _LIB = L
+    return _LIB
+
+
+class PGTError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        msg = lib().pgt_last_error().decode(errors="replace")
+        super().__init__("%s: %s (%s)" % (where, msg, ERRORS.get(code, code)))
+
+
+def _check(rc, where):
+    if rc != PGT_OK:
+        raise PGTError(rc, where)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return ctypes.c_void_p(a)
+    if hasattr(a, "data_ptr"):  # torch tensor on the device
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def device_count():
+    return lib().pgt_device_count()
+
+
+def poly12_params(v0=1.0, eps=0.2, rcut=1.25):
+    return [v0, eps, -28.0 * v0 / rcut ** 12, 48.0 * v0 / rcut ** 14, -21.0 * v0 / rcut ** 16]
+
+
+class Engine(object):
+    """One pgt_ik_engine."""
+
+    def __init__(self, dim=2, pot="poly12", params=None, scaled_rcut=1.25, cg="octic", cg_rcut=2.0, order=5,
+                 max_rcut=None, precision="fp64", diameters=None):
+        if params is None:
+            params = poly12_params(rcut=scaled_rcut)
+        if max_rcut is None:
+            if diameters is None:
+                raise ValueError("give max_rcut or the diameters it is computed from")
+            dmax = float(np.max(diameters))
+            max_rcut = max(scaled_rcut * dmax, dmax)
+        p = Params()
+        p.dim = dim
+        p.potential_kind = POT[pot]
+        p.scaled_rcut = scaled_rcut
+        p.n_pot_params = len(params)
+        for k, v in enumerate(params):
+            p.pot_params[k] = v
+        p.cg_kind = CG[cg]
+        p.cg_rcut = cg_rcut
+        p.quad_mode = 0
+        p.quad_order = order
+        p.precision = PREC[precision]
+        p.max_rcut = max_rcut
+        self.params = p
+        self.dim = dim
+        self.max_rcut = max_rcut
+        h = ctypes.c_void_p()
+        _check(lib().pgt_ik_create(ctypes.byref(p), ctypes.byref(h)), "pgt_ik_create")
+        self.h = h
+        self.G = 0
+
+    def close(self):
+        if self.h:
+            lib().pgt_ik_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_box(self, L, periodic, lower=None):
+        L = _f64(L)
+        lo = _f64(lower) if lower is not None else -0.5 * L
+        per = np.ascontiguousarray(periodic, dtype=np.int32)
+        _check(lib().pgt_ik_set_box(self.h, _ptr(L), _ptr(lo), _ptr(per)), "pgt_ik_set_box")
+
+    def set_particles(self, pos, diam, mass=None, vel=None, disp=None):
+        pos = _f64(pos, (-1, 3))
+        n = pos.shape[0]
+        self._keep = [pos, _f64(diam), None if mass is None else _f64(mass), None if vel is None else _f64(vel, (-1, 3)),
+                      None if disp is None else _f64(disp, (-1, 3))]
+        _check(lib().pgt_ik_set_particles(self.h, n, *[_ptr(a) for a in self._keep]), "pgt_ik_set_particles")
+
+    def set_grid(self, grid):
+        grid = _f64(grid, (-1, 3))
+        self.G = grid.shape[0]
+        _check(lib().pgt_ik_set_grid(self.h, self.G, _ptr(grid)), "pgt_ik_set_grid")
+
+    def set_system(self, system, grid):
+        self.set_box(system["L"], system["periodic"])
+        self.set_particles(system["pos"], system["diam"], system.get("mass"), system.get("vel"), system.get("disp"))
+        self.set_grid(grid)
+
+    def compute(self, fields=("virial",)):
+        D = self.dim
+        out = {}
+        mask = 0
+        ptrs = [None, None, None]
+        for k, (name, bit) in enumerate((("virial", FIELD_VIRIAL), ("kinetic", FIELD_KINETIC), ("elastic", FIELD_ELASTIC))):
+            if name in fields:
+                out[name] = np.zeros((self.G, D, D))
+                ptrs[k] = _ptr(out[name])
+                mask |= bit
+        _check(lib().pgt_ik_compute(self.h, mask, *ptrs), "pgt_ik_compute")
+        return out
+
+    def compute_frames(self, nframes, n, d_pos, d_diam, d_mass=None, d_vel=None, d_disp=None, fields=("virial",),
+                       d_virial=None, d_kinetic=None, d_elastic=None):
+        mask = (FIELD_VIRIAL if "virial" in fields else 0) | (FIELD_KINETIC if "kinetic" in fields else 0) | \
+               (FIELD_ELASTIC if "elastic" in fields else 0)
+        _check(lib().pgt_ik_compute_frames(self.h, nframes, n, _ptr(d_pos), _ptr(d_diam), _ptr(d_mass), _ptr(d_vel),
+                                           _ptr(d_disp), mask, _ptr(d_virial), _ptr(d_kinetic), _ptr(d_elastic)),
+               "pgt_ik_compute_frames")
+
+    def set_stream(self, stream_handle):
+        _check(lib().pgt_ik_set_stream(self.h, ctypes.c_void_p(stream_handle) if stream_handle else None),
+               "pgt_ik_set_stream")
+
+    def set_tuning(self, lanes_per_point=0, frames_per_launch=0):
+        _check(lib().pgt_ik_set_tuning(self.h, lanes_per_point, frames_per_launch), "pgt_ik_set_tuning")
+
+    def stats(self):
+        s = Stats()
+        _check(lib().pgt_ik_get_stats(self.h, ctypes.byref(s)), "pgt_ik_get_stats")
+        return s.as_dict()
+
+
+def eval_pair_force(rij, di, dj, pot="poly12", params=None, scaled_rcut=1.25):
+    if params is None:
+        params = poly12_params(rcut=scaled_rcut)
+    rij = _f64(rij, (-1, 3))
+    n = rij.shape[0]
+    di, dj = _f64(np.broadcast_to(di, (n,))), _f64(np.broadcast_to(dj, (n,)))
+    p = _f64(params)
+    f, e, r = np.zeros(n), np.zeros(n), np.zeros(n)
+    _check(lib().pgt_eval_pair_force(POT[pot], _ptr(p), len(params), scaled_rcut, n, _ptr(rij), _ptr(di), _ptr(dj),
+                                     _ptr(f), _ptr(e), _ptr(r)), "pgt_eval_pair_force")
+    return f, e, r
+
+
+def eval_cgfunc(mode, x, ri, rij=None, s=None, cg="octic", rcut=2.0, order=5):
+    x = _f64(x, (-1, 3))
+    n = x.shape[0]
+    ri = _f64(ri, (-1, 3))
+    rij = None if rij is None else _f64(rij, (-1, 3))
+    s = None if s is None else _f64(np.broadcast_to(s, (n,)))
+    out = np.zeros(n)
+    modes = {"delta": 0, "obj": 1, "bond": 2}
+    _check(lib().pgt_eval_cgfunc(CG[cg], rcut, 0, order, 0, 0.0, 0.0, modes[mode], n, _ptr(s), _ptr(x), _ptr(ri),
+                                 _ptr(rij), _ptr(out)), "pgt_eval_cgfunc")
+    return out
