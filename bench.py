@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Benchmark: Irving-Kirkwood stress fields, frames per second on one MI355X.
+
+Workload (one "step"): 100 frames of a 2D polydisperse12 glass, N = 65,536
+particles each (seeded lattice+jitter recipe, pyglasstools_amd/workload.py,
+seeds 2000..2099), all resident in HBM; FP64 virial + kinetic stress on a
+256 x 256 grid, Octic kernel r_cg = 2.0, 5-node Gauss-Legendre bond
+integral.  value = frames / second over K timed steps.
+
+Printed as one JSON line.  `roofline` is the field-gather kernel (the
+dominant kernel): algorithmic FP64 flops per launch (112 per pair-grid
+contribution U, 23 per kinetic (grid point, particle) pair, counted by the
+kernel itself) divided by its average launch time from HIP events on the
+engine's stream.  `cpu_baseline` is the FP64 CPU oracle (a C restatement of
+the reference loop) on the host cores over a bounded sample of the same
+frames.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "IK stress frames/sec at ≤1e-5 rel err vs FP64; surrogate vs plain quad"
+FLOP_U_2D, FLOP_KIN_2D = 112, 23
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def make_frames(nframes, n, config=2):
+    from pyglasstools_amd import workload
+
+    systems = [workload.make_system(n, 2, 1000 * config + f) for f in range(nframes)]
+    return systems
+
+
+def traffic_from_profile():
+    """HBM bytes per gather launch measured by rocprofv3 PMC counters, if a
+    committed summary exists (profiles/gather_c2_traffic.json)."""
+    p = os.path.join(ROOT, "profiles", "gather_c2_traffic.json")
+    if os.path.exists(p):
+        try:
+            with open(p) as f:
+                d = json.load(f)
+            return d.get("bytes_per_launch"), d.get("frames_per_launch")
+        except (OSError, ValueError):
+            pass
+    return None, None
+
+
+def cpu_baseline(systems, grid, max_rcut, seconds=10.0):
+    """Oracle on the host: frames in order until `seconds` of wall time."""
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    threads = max(1, min(threads, 16))
+    out = {}
+    for mode, name in ((1, "clean"), (0, "faithful")):
+        done, t0 = 0, time.perf_counter()
+        while done < len(systems):
+            oracle.ik_compute(systems[done], grid, fields=("virial", "kinetic"), max_rcut=max_rcut, mode=mode,
+                              nthreads=threads)
+            done += 1
+            if time.perf_counter() - t0 >= seconds:
+                break
+        dt = time.perf_counter() - t0
+        out[name] = {"frames": done, "seconds": dt, "fps": done / dt}
+    return out, threads
+
+
+def parity_sample(systems, grid, max_rcut, got_virial, got_kinetic, stride=16):
+    """Oracle vs the GPU result of frame 0 on every `stride`-th grid point."""
+    import oracle
+
+    sub = np.arange(0, len(grid), stride)
+    ref = oracle.ik_compute(systems[0], grid[sub], fields=("virial", "kinetic"), max_rcut=max_rcut, with_abs=True)
+    scale = np.maximum(np.abs(ref["virial"]), 1e-6 * ref["virial_abs"])
+    ev = float((np.abs(got_virial[sub] - ref["virial"]) / np.where(scale > 0, scale, 1)).max())
+    ek = float(np.abs(got_kinetic[sub] - ref["kinetic"]).max() / max(np.abs(ref["kinetic"]).max(), 1e-300))
+    rel = float(np.abs(got_virial[sub] - ref["virial"]).max() / np.abs(ref["virial"]).max())
+    return {"grid_points": int(len(sub)), "virial_max_scaled_err": ev, "virial_max_err_over_max": rel,
+            "kinetic_max_err_over_max": ek}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--frames", type=int, default=100)
+    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity sample")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--lpg", type=int, default=0, help="lanes per grid point (0: autotune)")
+    ap.add_argument("--fpl", type=int, default=0, help="frames per launch (0: autotune)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from pyglasstools_amd import engine, workload
+    from pyglasstools_amd.autotune import Autotuner
+
+    if engine.device_count() < 1 or not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    F, N = args.frames, args.n
+    log("generating %d frames of N=%d" % (F, N))
+    systems = make_frames(F, N)
+    L = systems[0]["L"]
+    grid = workload.cartesian_grid_2d(L[0], args.grid)
+    G = len(grid)
+    d0 = float(systems[0]["diam"].max())
+    max_rcut = max(1.25 * d0, d0)  # Calculator.h:41-45, from the first frame
+
+    pos = torch.tensor(np.stack([s["pos"] for s in systems]), dtype=torch.float64, device=dev)
+    diam = torch.tensor(np.stack([s["diam"] for s in systems]), dtype=torch.float64, device=dev)
+    mass = torch.tensor(np.stack([s["mass"] for s in systems]), dtype=torch.float64, device=dev)
+    vel = torch.tensor(np.stack([s["vel"] for s in systems]), dtype=torch.float64, device=dev)
+    out_v = torch.empty((F, G, 2, 2), dtype=torch.float64, device=dev)
+    out_k = torch.empty((F, G, 2, 2), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    eng = engine.Engine(dim=2, max_rcut=max_rcut, precision=args.precision)
+    eng.set_box(L, [1, 1, 0])
+    eng.set_grid(grid)
+
+    def step():
+        eng.compute_frames(F, N, pos, diam, mass, vel, None, fields=("virial", "kinetic"), d_virial=out_v,
+                           d_kinetic=out_k)
+
+    tuned, trials = None, []
+    if args.lpg and args.fpl:
+        eng.set_tuning(args.lpg, args.fpl)
+        tuned = {"lanes_per_point": args.lpg, "frames_per_launch": args.fpl}
+    else:
+        log("autotuning lanes per grid point x frames per launch")
+        tuned, trials = Autotuner().tune(eng, step, ("c2", N, G, F, args.precision), F, sync=torch.cuda.synchronize)
+        log("tuned: %s" % tuned)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    gather_ms, n_contrib, n_ball, launches = 0.0, 0, 0, 0
+    fpl = tuned["frames_per_launch"]
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        st = eng.stats()
+        gather_ms += st["gather_ms"]
+        n_contrib += st["n_contrib"]
+        n_ball += st["n_ball"]
+        launches += (F + fpl - 1) // fpl
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    fps = F * args.steps / elapsed
+    log("timed %d steps in %.3f s: %.1f frames/s" % (args.steps, elapsed, fps))
+
+    flops = FLOP_U_2D * n_contrib + FLOP_KIN_2D * n_ball
+    avg_launch_s = gather_ms / 1e3 / launches
+    achieved = flops / launches / avg_launch_s / 1e12
+    traffic, traffic_fpl = traffic_from_profile()
+    if traffic is not None and traffic_fpl != fpl:
+        traffic = traffic * fpl / traffic_fpl  # per-launch bytes scale with frames per launch
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+                "kernel": "k_gather (field gather)", "compute": "FP64 VALU (published MI355X FP64 peak 78.6 TFLOP/s)",
+                "avg_launch_ms": round(avg_launch_s * 1e3, 4), "flops_per_launch": flops / launches}
+
+    extra = {"frames": F, "n_particles": N, "grid_points": G, "contributions_per_s": n_contrib / elapsed,
+             "contributions_per_frame": n_contrib / (F * args.steps), "kinetic_pairs_per_frame": n_ball / (F * args.steps),
+             "gather_share_of_step": gather_ms / 1e3 / elapsed, "tuning": tuned, "precision": args.precision}
+    if trials:
+        extra["autotune_trials"] = trials
+
+    cpu = None
+    if not args.no_cpu:
+        step()
+        torch.cuda.synchronize()
+        extra["parity_frame0"] = parity_sample(systems, grid, max_rcut, out_v[0].cpu().numpy(), out_k[0].cpu().numpy())
+        log("parity sample: %s" % extra["parity_frame0"])
+        res, threads = cpu_baseline(systems, grid, max_rcut, args.cpu_seconds)
+        log("cpu oracle: %s" % res)
+        cpu = {"value": round(res["clean"]["fps"], 4), "unit": "frames/s", "cores": threads, "kind": "port",
+               "sample": "%d of the %d frames, full 256x256 grid, FP64 oracle (C restatement of "
+                         "IrvingKirkwood::compute, OpenMP over grid points, no per-triple allocations)"
+                         % (res["clean"]["frames"], F)}
+        extra["cpu_faithful_fps"] = res["faithful"]["fps"]
+        extra["cpu_faithful_sample_frames"] = res["faithful"]["frames"]
+        extra["speedup_vs_cpu_clean"] = fps / res["clean"]["fps"]
+        extra["speedup_vs_cpu_faithful"] = fps / res["faithful"]["fps"]
+
+    line = {"metric": METRIC, "value": round(fps, 3), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+            "data": "synthetic (seeded lattice+jitter polydisperse glass, 100 frames resident in HBM)",
+            "config": {"workload": "2D polydisperse12 N=65536, 100 frames, 256x256 grid, Octic r_cg=2.0, GL5, "
+                                   "FP64 virial + kinetic stress", "frames_per_step": F, "n_particles": N,
+                       "grid": "%dx%d" % (args.grid, args.grid), "parallelism": "single GPU"},
+            "roofline": roofline, "cpu_baseline": cpu, "extra": extra}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,69 @@
+"""Autotuner for the field-gather launch shape.
+
+Two knobs of the engine are tuned per workload: lanes per grid point (1, 4
+or 8 lanes share one grid point's candidate particles and are combined by a
+wavefront shuffle reduction) and frames per launch (how many frames one
+kernel launch covers).  Each candidate is timed on the caller's own job; the
+best is cached per (device, workload key) in memory and, when a path is
+given, in a JSON file.
+"""
+import json
+import os
+import time
+
+DEFAULT_LPG = (1, 4, 8)
+DEFAULT_FPL = (4, 10, 25, 50, 100)
+
+
+def _key(parts):
+    return "|".join(str(p) for p in parts)
+
+
+class Autotuner(object):
+    def __init__(self, cache_path=None):
+        self.cache_path = cache_path
+        self.cache = {}
+        if cache_path and os.path.exists(cache_path):
+            try:
+                with open(cache_path) as f:
+                    self.cache = json.load(f)
+            except (OSError, ValueError):
+                self.cache = {}
+
+    def tune(self, engine, run_job, key, nframes, lpg=DEFAULT_LPG, fpl=DEFAULT_FPL, sync=None, repeats=2):
+        """run_job() runs the whole job once; sync() waits for the device.
+        Returns (best settings dict, list of trial records)."""
+        k = _key(key)
+        if k in self.cache:
+            best = self.cache[k]
+            engine.set_tuning(best["lanes_per_point"], best["frames_per_launch"])
+            return best, []
+        trials = []
+        for l in lpg:
+            for fp in sorted(set(min(f, nframes) for f in fpl)):
+                engine.set_tuning(l, fp)
+                run_job()  # first call allocates for this shape
+                if sync:
+                    sync()
+                t = []
+                for _ in range(repeats):
+                    t0 = time.perf_counter()
+                    run_job()
+                    if sync:
+                        sync()
+                    t.append(time.perf_counter() - t0)
+                trials.append({"lanes_per_point": l, "frames_per_launch": fp, "seconds": min(t)})
+        best = min(trials, key=lambda r: r["seconds"])
+        best = {"lanes_per_point": best["lanes_per_point"], "frames_per_launch": best["frames_per_launch"]}
+        engine.set_tuning(best["lanes_per_point"], best["frames_per_launch"])
+        self.cache[k] = best
+        if self.cache_path:
+            try:
+                d = os.path.dirname(self.cache_path)
+                if d:
+                    os.makedirs(d, exist_ok=True)
+                with open(self.cache_path, "w") as f:
+                    json.dump(self.cache, f, indent=1, sort_keys=True)
+            except OSError:
+                pass
+        return best, trials
