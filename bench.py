@@ -102,9 +102,24 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity sample")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--lpg", type=int, default=0, help="lanes per grid point (0: autotune)")
+    ap.add_argument("--block", type=int, default=0, help="gather threads per workgroup (0: autotune)")
     ap.add_argument("--fpl", type=int, default=0, help="frames per launch (0: autotune)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 3 device copies of MIB MiB (reference dispatches for HBM byte counters) and exit")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -147,9 +162,9 @@ def main():
                            d_kinetic=out_k)
 
     tuned, trials = None, []
-    if args.lpg and args.fpl:
-        eng.set_tuning(args.lpg, args.fpl)
-        tuned = {"lanes_per_point": args.lpg, "frames_per_launch": args.fpl}
+    if args.block and args.fpl:
+        eng.set_tuning(args.block, args.fpl)
+        tuned = {"threads_per_block": args.block, "frames_per_launch": args.fpl}
     else:
         log("autotuning lanes per grid point x frames per launch")
         tuned, trials = Autotuner().tune(eng, step, ("c2", N, G, F, args.precision), F, sync=torch.cuda.synchronize)

@@ -154,9 +154,9 @@ int pgt_ik_compute_frames(pgt_ik_engine *e, int nframes, int64_t n, const double
 /* Run the engine's work on the given hipStream_t (NULL: the engine's own). */
 int pgt_ik_set_stream(pgt_ik_engine *e, void *hip_stream);
 
-/* Lanes per grid point (1,2,4,8,16) and frames per launch used by the gather
- * kernel; 0 keeps the current choice.  Used by the autotuner. */
-int pgt_ik_set_tuning(pgt_ik_engine *e, int lanes_per_point, int frames_per_launch);
+/* Threads per workgroup of the field-gather kernel (128 or 256) and frames
+ * per launch; 0 keeps the current choice.  Used by the autotuner. */
+int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_launch);
 
 int pgt_ik_get_stats(pgt_ik_engine *e, pgt_ik_stats *out);
 

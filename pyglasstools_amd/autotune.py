@@ -1,17 +1,17 @@
 """Autotuner for the field-gather launch shape.
 
-Two knobs of the engine are tuned per workload: lanes per grid point (1, 4
-or 8 lanes share one grid point's candidate particles and are combined by a
-wavefront shuffle reduction) and frames per launch (how many frames one
-kernel launch covers).  Each candidate is timed on the caller's own job; the
-best is cached per (device, workload key) in memory and, when a path is
-given, in a JSON file.
+Two knobs of the engine are tuned per workload: threads per workgroup of the
+gather kernel (128 or 256; with the per-lane LDS lists this sets how many
+workgroups share a CU) and frames per launch (how many frames one kernel
+launch covers; more frames amortise launch gaps, fewer keep the work arrays
+small).  Each candidate is timed on the caller's own job; the best is cached
+per workload key in memory and, when a path is given, in a JSON file.
 """
 import json
 import os
 import time
 
-DEFAULT_LPG = (1, 4, 8)
+DEFAULT_BLOCK = (128, 256)
 DEFAULT_FPL = (4, 10, 25, 50, 100)
 
 
@@ -30,18 +30,18 @@ class Autotuner(object):
             except (OSError, ValueError):
                 self.cache = {}
 
-    def tune(self, engine, run_job, key, nframes, lpg=DEFAULT_LPG, fpl=DEFAULT_FPL, sync=None, repeats=2):
+    def tune(self, engine, run_job, key, nframes, blocks=DEFAULT_BLOCK, fpl=DEFAULT_FPL, sync=None, repeats=2):
         """run_job() runs the whole job once; sync() waits for the device.
         Returns (best settings dict, list of trial records)."""
         k = _key(key)
         if k in self.cache:
             best = self.cache[k]
-            engine.set_tuning(best["lanes_per_point"], best["frames_per_launch"])
+            engine.set_tuning(best["threads_per_block"], best["frames_per_launch"])
             return best, []
         trials = []
-        for l in lpg:
+        for b in blocks:
             for fp in sorted(set(min(f, nframes) for f in fpl)):
-                engine.set_tuning(l, fp)
+                engine.set_tuning(b, fp)
                 run_job()  # first call allocates for this shape
                 if sync:
                     sync()
@@ -52,10 +52,10 @@ class Autotuner(object):
                     if sync:
                         sync()
                     t.append(time.perf_counter() - t0)
-                trials.append({"lanes_per_point": l, "frames_per_launch": fp, "seconds": min(t)})
+                trials.append({"threads_per_block": b, "frames_per_launch": fp, "seconds": min(t)})
         best = min(trials, key=lambda r: r["seconds"])
-        best = {"lanes_per_point": best["lanes_per_point"], "frames_per_launch": best["frames_per_launch"]}
-        engine.set_tuning(best["lanes_per_point"], best["frames_per_launch"])
+        best = {"threads_per_block": best["threads_per_block"], "frames_per_launch": best["frames_per_launch"]}
+        engine.set_tuning(best["threads_per_block"], best["frames_per_launch"])
         self.cache[k] = best
         if self.cache_path:
             try:

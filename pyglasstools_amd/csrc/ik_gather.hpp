@@ -1,0 +1,298 @@
+// ik_gather.hpp -- the field-gather kernel of the IK stress-field engine.
+// Included by ik_kernels.hpp.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "ik_device.hpp"
+
+namespace pgt {
+
+struct GatherOut {
+    double *vir;
+    double *kin;
+    double *ela;
+    unsigned long long *n_ball;
+    unsigned long long *n_contrib;
+};
+
+// sum_k w_k shape(t(s_k)), t(s) = A + s B + s^2 C: the bond integral of the
+// unnormalised kernel along the segment, in units of the kernel width.
+template <int CG, int NQ, typename ER>
+__device__ __forceinline__ ER bond_quad(const GLRule &R, const double *__restrict__ gld, int nq, ER A, ER B, ER C)
+{
+    ER acc = ER(0);
+    if constexpr (NQ > 0) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            const ER s = (ER)R.s[k];
+            const ER t = fma(s, fma(s, C, B), A);
+            acc = fma((ER)R.w[k], shape<CG, ER>(t), acc);
+        }
+    } else {
+        for (int k = 0; k < nq; ++k) {
+            const ER s = (ER)gld[k];
+            const ER t = fma(s, fma(s, C, B), A);
+            acc = fma((ER)gld[nq + k], shape<CG, ER>(t), acc);
+        }
+    }
+    return acc;
+}
+
+// One bond entry (ES values, a 16-byte multiple) in 16-byte loads.
+template <typename ER, int ES>
+__device__ __forceinline__ void load_entry(const ER *__restrict__ p, ER (&v)[ES])
+{
+    using V = typename std::conditional<std::is_same<ER, double>::value, double2, float4>::type;
+    constexpr int PER = 16 / (int)sizeof(ER);
+    const V *q = reinterpret_cast<const V *>(p);
+#pragma unroll
+    for (int k = 0; k < ES / PER; ++k) {
+        const V t = q[k];
+        if constexpr (PER == 2) {
+            v[2 * k] = t.x;
+            v[2 * k + 1] = t.y;
+        } else {
+            v[4 * k] = t.x;
+            v[4 * k + 1] = t.y;
+            v[4 * k + 2] = t.z;
+            v[4 * k + 3] = t.w;
+        }
+    }
+}
+
+// Field gather, one lane per grid point (grid points taken in the order of
+// `perm`, which puts nearby points in the same wavefront).
+//
+// Pass A scans the cells around the grid point and compacts the particles
+// inside the coarse-graining ball, with their bond counts, into a per-lane
+// list in LDS (one int per entry, index | count << 26, when PACK is set:
+// fewer than 2^26 particles and at most 63 bonds each; otherwise a second
+// array holds the counts).  The list is then sorted by bond count (largest first), so
+// that at every step of pass B the lanes of a wavefront hold particles with
+// similar bond counts and the bond loop wastes few lanes.  Pass B adds each
+// particle's kinetic term and integrates its bonds; the next bond entry and
+// the next particle's data are loaded before the current ones are used, so
+// memory latency overlaps the arithmetic.  A full list is processed early.
+// All sums are FP64 in a fixed order.
+template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, typename ER, int BLK, int MAXL, bool PACK>
+__global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const double *__restrict__ grid,
+                                                const int *__restrict__ perm, GLRule R,
+                                                const double *__restrict__ gld, double cgnorm,
+                                                const int *__restrict__ cstart, const double *__restrict__ spos,
+                                                const double *__restrict__ skin, int cap,
+                                                const int *__restrict__ bcount, const ER *__restrict__ bonds,
+                                                GatherOut out)
+{
+    constexpr int PS = (D == 2) ? 2 : 4;
+    constexpr int KN = OD * (OD + 1) / 2;
+    constexpr bool PAIRS = VIR || ELA;
+    constexpr int ES = BondLayout<D, OD, ELA, ER>::ES;
+    __shared__ int lp[MAXL][BLK];
+    __shared__ unsigned short lnb[(PAIRS && !PACK) ? MAXL : 1][BLK];
+
+    const int f = blockIdx.y;
+    const int lane = threadIdx.x;
+    // list entry k of this lane: particle index and bond count
+    auto getp = [&](int k) -> int { return PACK ? (lp[k][lane] & 0x3FFFFFF) : lp[k][lane]; };
+    auto getnb = [&](int k) -> int {
+        if constexpr (!PAIRS) return 0;
+        else if constexpr (PACK) return (int)((unsigned)lp[k][lane] >> 26);
+        else return lnb[k][lane];
+    };
+    const int t = blockIdx.x * BLK + lane;
+    const bool live = t < G;
+    const bool KIN = skin != nullptr;
+    const double *sp = spos + (size_t)f * n * PS;
+    const int *st = cstart + (size_t)f * (g.ncell + 1);
+    const int *bc = bcount + (size_t)f * n;
+    const ER *bd = bonds + (size_t)f * n * (size_t)cap * ES;
+    const double *sk = KIN ? skin + (size_t)f * n * KN : nullptr;
+
+    double av[KN], ae[KN], ak[KN];
+#pragma unroll
+    for (int k = 0; k < KN; ++k) { av[k] = 0.0; ae[k] = 0.0; ak[k] = 0.0; }
+    unsigned long long nball = 0, ncontrib = 0;
+    const int gp = live ? perm[t] : 0;
+
+    if (live) {
+        const double x0 = grid[(size_t)gp * 3], x1 = grid[(size_t)gp * 3 + 1];
+        const double x2 = (D == 3) ? grid[(size_t)gp * 3 + 2] : 0.0;
+        const double s2 = 2.0 * g.inv_rc;
+
+        auto integrate = [&](const ER (&e)[ES], ER ta0, ER ta1, ER ta2, ER A) {
+            ER B = ta0 * e[0] + ta1 * e[1];
+            if (D == 3) B = fma(ta2, e[2], B);
+            const ER w = bond_quad<CG, NQ, ER>(R, gld, R.n, A, B, e[D]);
+            if (VIR) {
+                const ER c = w * e[D + 1];
+                const ER cb0 = c * e[0], cb1 = c * e[1];
+                if constexpr (std::is_same<ER, double>::value) {
+                    av[0] = fma(cb0, e[0], av[0]);
+                    av[1] = fma(cb1, e[1], av[1]);
+                    if (OD == 2) {
+                        av[2] = fma(cb0, e[1], av[2]);
+                    } else {
+                        const ER cb2 = c * e[2];
+                        av[2] = fma(cb2, e[2], av[2]);
+                        av[3] = fma(cb0, e[1], av[3]);
+                        av[4] = fma(cb0, e[2], av[4]);
+                        av[5] = fma(cb1, e[2], av[5]);
+                    }
+                } else {
+                    av[0] += (double)(cb0 * e[0]);
+                    av[1] += (double)(cb1 * e[1]);
+                    if (OD == 2) {
+                        av[2] += (double)(cb0 * e[1]);
+                    } else {
+                        const ER cb2 = c * e[2];
+                        av[2] += (double)(cb2 * e[2]);
+                        av[3] += (double)(cb0 * e[1]);
+                        av[4] += (double)(cb0 * e[2]);
+                        av[5] += (double)(cb1 * e[2]);
+                    }
+                }
+            }
+            if (ELA) {
+#pragma unroll
+                for (int m = 0; m < KN; ++m) {
+                    if constexpr (std::is_same<ER, double>::value) ae[m] = fma(w, e[D + 2 + m], ae[m]);
+                    else ae[m] += (double)(w * e[D + 2 + m]);
+                }
+            }
+        };
+
+        // pass B over the first cnt list entries
+        auto process = [&](int cnt) {
+            if constexpr (PAIRS) {
+                // stable insertion sort by bond count, largest first
+                for (int k = 1; k < cnt; ++k) {
+                    const int ve = lp[k][lane];
+                    const int vn = getnb(k);
+                    const unsigned short vs = PACK ? 0 : lnb[k][lane];
+                    int m = k - 1;
+                    while (m >= 0 && getnb(m) < vn) {
+                        lp[m + 1][lane] = lp[m][lane];
+                        if constexpr (!PACK) lnb[m + 1][lane] = lnb[m][lane];
+                        --m;
+                    }
+                    lp[m + 1][lane] = ve;
+                    if constexpr (!PACK) lnb[m + 1][lane] = vs;
+                }
+            }
+            int p = getp(0);
+            double pos[PS], kk[KN];
+#pragma unroll
+            for (int a = 0; a < PS; ++a) pos[a] = sp[(size_t)p * PS + a];
+            if (KIN)
+#pragma unroll
+                for (int k = 0; k < KN; ++k) kk[k] = sk[(size_t)p * KN + k];
+            for (int q = 0; q < cnt; ++q) {
+                // next particle's data, loaded ahead
+                const int pn = getp(q + 1 < cnt ? q + 1 : q);
+                double npos[PS], nkk[KN];
+#pragma unroll
+                for (int a = 0; a < PS; ++a) npos[a] = sp[(size_t)pn * PS + a];
+                if (KIN)
+#pragma unroll
+                    for (int k = 0; k < KN; ++k) nkk[k] = sk[(size_t)pn * KN + k];
+
+                const double a0 = minimg(pos[0] - x0, g.L[0], g.invL[0], g.periodic[0]);
+                const double a1 = minimg(pos[1] - x1, g.L[1], g.invL[1], g.periodic[1]);
+                const double a2 = (D == 3) ? minimg(pos[2] - x2, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                const double A = (a0 * a0 + a1 * a1 + a2 * a2) * g.inv_rc2;
+                if (KIN) {
+                    const double w = shape<CG, double>(A);
+#pragma unroll
+                    for (int k = 0; k < KN; ++k) ak[k] = fma(w, kk[k], ak[k]);
+                }
+                if constexpr (PAIRS) {
+                    const int nb = getnb(q);
+                    ncontrib += nb;
+                    if (nb > 0) {
+                        const ER ta0 = (ER)(a0 * s2), ta1 = (ER)(a1 * s2), ta2 = (ER)(a2 * s2), Ae = (ER)A;
+                        const ER *ptr = bd + (size_t)p * cap * ES;
+                        ER E[ES];
+                        load_entry<ER, ES>(ptr, E);
+                        for (int j = 0; j < nb; ++j) {
+                            ER En[ES];
+                            load_entry<ER, ES>(ptr + (size_t)(j + 1 < nb ? j + 1 : j) * ES, En);
+                            integrate(E, ta0, ta1, ta2, Ae);
+#pragma unroll
+                            for (int k = 0; k < ES; ++k) E[k] = En[k];
+                        }
+                    }
+                }
+                p = pn;
+#pragma unroll
+                for (int a = 0; a < PS; ++a) pos[a] = npos[a];
+                if (KIN)
+#pragma unroll
+                    for (int k = 0; k < KN; ++k) kk[k] = nkk[k];
+            }
+        };
+
+        // pass A
+        const int cx = cell_coord(g, 0, x0), cy = cell_coord(g, 1, x1), cz = cell_coord(g, 2, x2);
+        int rows[9];
+        const int nrows = neighbour_rows(g, cy, cz, rows);
+        int cnt = 0;
+        for (int r = 0; r < nrows; ++r) {
+            int rb[2], re[2];
+            const int nr = row_ranges(g, st, rows[r], cx, rb, re);
+            for (int s = 0; s < nr; ++s) {
+                for (int p = rb[s]; p < re[s]; ++p) {
+                    const double a0 = minimg(sp[(size_t)p * PS] - x0, g.L[0], g.invL[0], g.periodic[0]);
+                    const double a1 = minimg(sp[(size_t)p * PS + 1] - x1, g.L[1], g.invL[1], g.periodic[1]);
+                    const double a2 = (D == 3) ? minimg(sp[(size_t)p * PS + 2] - x2, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                    const double asq = a0 * a0 + a1 * a1 + a2 * a2;
+                    if (!(asq < g.rc2)) continue;
+                    ++nball;
+                    if constexpr (PAIRS) {
+                        const int nb = min(bc[p], cap);
+                        if constexpr (PACK) {
+                            lp[cnt][lane] = p | (nb << 26);
+                        } else {
+                            lp[cnt][lane] = p;
+                            lnb[cnt][lane] = (unsigned short)nb;
+                        }
+                    } else {
+                        lp[cnt][lane] = p;
+                    }
+                    ++cnt;
+                    if (cnt == MAXL) {
+                        process(cnt);
+                        cnt = 0;
+                    }
+                }
+            }
+        }
+        if (cnt > 0) process(cnt);
+
+        const size_t ob = ((size_t)f * G + gp) * OD * OD;
+        auto store = [&](double *dst, const double *acc, double scale) {
+            for (int a = 0; a < OD; ++a) dst[ob + a + OD * a] = acc[a] * scale;
+            if (OD == 2) {
+                dst[ob + 1] = acc[2] * scale;
+                dst[ob + 2] = acc[2] * scale;
+            } else {
+                dst[ob + 1] = dst[ob + 3] = acc[3] * scale;
+                dst[ob + 2] = dst[ob + 6] = acc[4] * scale;
+                dst[ob + 5] = dst[ob + 7] = acc[5] * scale;
+            }
+        };
+        if (VIR) store(out.vir, av, 1.0);
+        if (ELA) store(out.ela, ae, 1.0);
+        if (KIN) store(out.kin, ak, cgnorm);
+    }
+    // counts: one atomic per wavefront
+    nball = wave_sum(nball);
+    ncontrib = wave_sum(ncontrib);
+    if ((lane & 63) == 0) {
+        atomicAdd(out.n_ball, nball);
+        atomicAdd(out.n_contrib, ncontrib);
+    }
+}
+
+}  // namespace pgt

@@ -1,0 +1,316 @@
+// ik_kernels.hpp -- the GPU kernels of the IK stress-field engine (gfx950).
+// Included once, by ik_engine.hip.
+//
+//   cell list   k_cell_count -> k_cell_scan -> k_cell_fill -> k_cell_rank
+//   bond list   k_bonds
+//   fields      k_gather
+//
+// All per-frame arrays are stacked along blockIdx.y (one frame per y index).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "ik_device.hpp"
+
+namespace pgt {
+
+// Geometry shared by all kernels of one launch.
+struct Geom {
+    double L[3], invL[3], lo[3];
+    int periodic[3];
+    int nc[3];     // cells per axis (1 on non-periodic axes and when < 3 fit)
+    double cs[3];  // cell edge
+    int ncell;
+    double rc2;      // cg_rcut^2
+    double inv_rc;   // 1/cg_rcut
+    double inv_rc2;  // 1/cg_rcut^2
+    double mr2;      // max_rcut^2
+};
+
+// Minimum image with the image index rint(d/L) taken as rint(d * (1/L)).
+// The two agree except when d/L is within a few ulp of a half-integer, i.e.
+// for separations of half a box, which every search here rejects (search
+// radii are required to be below half a periodic box).
+__device__ __forceinline__ double minimg(double d, double L, double invL, int periodic)
+{
+    return periodic ? fma(-L, rint(d * invL), d) : d;
+}
+
+__device__ __forceinline__ int cell_coord(const Geom &g, int a, double p)
+{
+    if (g.nc[a] == 1) return 0;
+    double u = p - g.lo[a];
+    u -= g.L[a] * floor(u * g.invL[a]);
+    int k = (int)floor(u / g.cs[a]);
+    return k < 0 ? 0 : (k >= g.nc[a] ? g.nc[a] - 1 : k);
+}
+
+// Up to two contiguous particle ranges covering cells [cx-1, cx+1] of one
+// row of the cell grid along x (wrapping periodically).
+__device__ __forceinline__ int row_ranges(const Geom &g, const int *__restrict__ cstart, int rowbase, int cx,
+                                          int *beg, int *end)
+{
+    const int nx = g.nc[0];
+    if (nx < 3) {
+        beg[0] = cstart[rowbase];
+        end[0] = cstart[rowbase + nx];
+        return 1;
+    }
+    const int lo = cx - 1, hi = cx + 1;
+    if (lo >= 0 && hi < nx) {
+        beg[0] = cstart[rowbase + lo];
+        end[0] = cstart[rowbase + hi + 1];
+        return 1;
+    }
+    if (lo < 0) {  // cells nx-1 and [0, 1]
+        beg[0] = cstart[rowbase + nx - 1];
+        end[0] = cstart[rowbase + nx];
+        beg[1] = cstart[rowbase];
+        end[1] = cstart[rowbase + 2];
+        return 2;
+    }
+    // hi == nx: cells [nx-2, nx-1] and 0
+    beg[0] = cstart[rowbase + nx - 2];
+    end[0] = cstart[rowbase + nx];
+    beg[1] = cstart[rowbase];
+    end[1] = cstart[rowbase + 1];
+    return 2;
+}
+
+// Base indices of the (up to 9) rows of cells around cell (cy, cz).
+__device__ __forceinline__ int neighbour_rows(const Geom &g, int cy, int cz, int *rowbase)
+{
+    const int ny = g.nc[1], nz = g.nc[2];
+    int ys[3], zs[3], nyy, nzz;
+    if (ny >= 3) { ys[0] = (cy + ny - 1) % ny; ys[1] = cy; ys[2] = (cy + 1) % ny; nyy = 3; }
+    else { for (int k = 0; k < ny; ++k) ys[k] = k; nyy = ny; }
+    if (nz >= 3) { zs[0] = (cz + nz - 1) % nz; zs[1] = cz; zs[2] = (cz + 1) % nz; nzz = 3; }
+    else { for (int k = 0; k < nz; ++k) zs[k] = k; nzz = nz; }
+    int n = 0;
+    for (int a = 0; a < nzz; ++a)
+        for (int b = 0; b < nyy; ++b) rowbase[n++] = (zs[a] * ny + ys[b]) * g.nc[0];
+    return n;
+}
+
+// Sum over the 64 lanes of a wavefront (all lanes must be active).
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ------------------------------------------------------------------------
+// cell list
+// ------------------------------------------------------------------------
+__global__ void k_cell_count(Geom g, int n, const double *__restrict__ pos, int *__restrict__ pcell,
+                             int *__restrict__ pslot, int *__restrict__ ccount)
+{
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *p = pos + ((size_t)f * n + i) * 3;
+    const int cx = cell_coord(g, 0, p[0]), cy = cell_coord(g, 1, p[1]), cz = cell_coord(g, 2, p[2]);
+    const int c = (cz * g.nc[1] + cy) * g.nc[0] + cx;
+    pcell[(size_t)f * n + i] = c;
+    pslot[(size_t)f * n + i] = atomicAdd(&ccount[(size_t)f * g.ncell + c], 1);
+}
+
+// Exclusive scan of the cell counts into cstart (ncell+1 entries per frame),
+// one 1024-thread workgroup per frame; the counts are cleared for reuse.
+__global__ __launch_bounds__(1024) void k_cell_scan(int ncell, int *__restrict__ ccount, int *__restrict__ cstart)
+{
+    const int f = blockIdx.x;
+    int *cnt = ccount + (size_t)f * ncell;
+    int *st = cstart + (size_t)f * (ncell + 1);
+    const int T = blockDim.x, t = threadIdx.x;
+    const int chunk = (ncell + T - 1) / T;
+    const int b0 = min(ncell, t * chunk), b1 = min(ncell, b0 + chunk);
+    int s = 0;
+    for (int k = b0; k < b1; ++k) s += cnt[k];
+    __shared__ int sh[1024];
+    sh[t] = s;
+    __syncthreads();
+    for (int off = 1; off < T; off <<= 1) {
+        const int v = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += v;
+        __syncthreads();
+    }
+    int run = sh[t] - s;
+    for (int k = b0; k < b1; ++k) {
+        st[k] = run;
+        run += cnt[k];
+        cnt[k] = 0;
+    }
+    if (t == T - 1) st[ncell] = sh[T - 1];
+}
+
+__global__ void k_cell_fill(int n, int ncell, const int *__restrict__ pcell, const int *__restrict__ pslot,
+                            const int *__restrict__ cstart, int *__restrict__ order)
+{
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t fi = (size_t)f * n + i;
+    order[(size_t)f * n + cstart[(size_t)f * (ncell + 1) + pcell[fi]] + pslot[fi]] = i;
+}
+
+// Place particle i at its cell start + (number of cell members with a
+// smaller input index), so the order inside a cell is the input order, and
+// write the cell-sorted particle arrays.
+template <int D, int OD>
+__global__ void k_cell_rank(int n, int ncell, const int *__restrict__ pcell, const int *__restrict__ cstart,
+                            const int *__restrict__ order, const double *__restrict__ pos,
+                            const double *__restrict__ diam, const double *__restrict__ mass,
+                            const double *__restrict__ vel, const double *__restrict__ disp,
+                            double *__restrict__ spos, double *__restrict__ sdiam, double *__restrict__ skin,
+                            double *__restrict__ sdisp)
+{
+    constexpr int PS = (D == 2) ? 2 : 4;
+    constexpr int KN = OD * (OD + 1) / 2;
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t fi = (size_t)f * n + i;
+    const int c = pcell[fi];
+    const int *st = cstart + (size_t)f * (ncell + 1);
+    const int *ord = order + (size_t)f * n;
+    const int b = st[c], e = st[c + 1];
+    int rank = 0;
+    for (int k = b; k < e; ++k) rank += ord[k] < i;
+    const size_t fk = (size_t)f * n + b + rank;
+    const double *p = pos + fi * 3;
+    double *sp = spos + fk * PS;
+    sp[0] = p[0];
+    sp[1] = p[1];
+    if (D == 3) { sp[2] = p[2]; sp[3] = 0.0; }
+    sdiam[fk] = diam[fi];
+    if (skin) {
+        const double m = mass ? mass[fi] : 1.0;
+        double v[3] = {0, 0, 0};
+        if (vel) { v[0] = vel[fi * 3]; v[1] = vel[fi * 3 + 1]; v[2] = vel[fi * 3 + 2]; }
+        double *kk = skin + fk * KN;
+        // symmetric packing: diagonal, then (0,1) [, (0,2), (1,2)]
+        for (int a = 0; a < OD; ++a) kk[a] = m * v[a] * v[a];
+        if (OD == 2) kk[2] = m * v[0] * v[1];
+        else { kk[3] = m * v[0] * v[1]; kk[4] = m * v[0] * v[2]; kk[5] = m * v[1] * v[2]; }
+    }
+    if (sdisp)
+        for (int a = 0; a < 3; ++a) sdisp[fk * 3 + a] = disp ? disp[fi * 3 + a] : 0.0;
+}
+
+// ------------------------------------------------------------------------
+// bond list
+// ------------------------------------------------------------------------
+// Entries of particle p (cell-sorted index) sit at bonds[(f*n + p)*cap*ES ...],
+// one per neighbour j within max_rcut with a nonzero pair force (or, with the
+// elastic field, a nonzero force at r_ij + u_ij).  Layout, in units of ER:
+//   [0, D)     b' = r_ij / rc                   (scaled bond vector)
+//   [D]        C' = |r_ij|^2 / rc^2
+//   [D+1]      c' = 0.5 * N_cg * (F/r) * rc^2   (virial: T += c' w b' (x) b')
+//   [D+2, ..)  elastic: 0.5 * N_cg * (F1_a (u+r)_b - F_a r_b), packed symmetric
+// padded to a 16-byte multiple.  N_cg is the kernel normalisation and w the
+// quadrature sum of the unnormalised kernel shape.
+template <int D, int OD, bool ELA, typename ER>
+struct BondLayout {
+    static constexpr int KN = OD * (OD + 1) / 2;
+    static constexpr int RAW = D + 2 + (ELA ? KN : 0);
+    static constexpr int PER16 = 16 / (int)sizeof(ER);
+    static constexpr int ES = ((RAW + PER16 - 1) / PER16) * PER16;
+};
+
+template <int D, int OD, bool VIR, bool ELA, typename ER>
+__global__ __launch_bounds__(256) void k_bonds(Geom g, int n, PotParams P, double cgnorm,
+                                               const int *__restrict__ cstart, const double *__restrict__ spos,
+                                               const double *__restrict__ sdiam, const double *__restrict__ sdisp,
+                                               int cap, int *__restrict__ bcount, ER *__restrict__ bonds,
+                                               int *__restrict__ maxcount)
+{
+    constexpr int PS = (D == 2) ? 2 : 4;
+    using LY = BondLayout<D, OD, ELA, ER>;
+    constexpr int ES = LY::ES;
+    const int f = blockIdx.y;
+    const int p_raw = blockIdx.x * blockDim.x + threadIdx.x;
+    // lanes past the end shadow the last particle without writing, so the
+    // whole wavefront takes part in the max-reduction at the end
+    const bool live = p_raw < n;
+    const int p = live ? p_raw : n - 1;
+    const double *sp = spos + (size_t)f * n * PS;
+    const double *sd = sdiam + (size_t)f * n;
+    const int *st = cstart + (size_t)f * (g.ncell + 1);
+    const double xi = sp[(size_t)p * PS], yi = sp[(size_t)p * PS + 1], zi = (D == 3) ? sp[(size_t)p * PS + 2] : 0.0;
+    const double di = sd[p];
+    const int cx = cell_coord(g, 0, xi), cy = cell_coord(g, 1, yi), cz = cell_coord(g, 2, zi);
+    int rows[9];
+    const int nrows = neighbour_rows(g, cy, cz, rows);
+    ER *out = bonds + ((size_t)f * n + p) * (size_t)cap * ES;
+    const double h = 0.5 * cgnorm;
+    int cnt = 0;
+    for (int r = 0; r < nrows; ++r) {
+        int rb[2], re[2];
+        const int nr = row_ranges(g, st, rows[r], cx, rb, re);
+        for (int s = 0; s < nr; ++s) {
+            if (rb[s] >= re[s]) continue;
+            // candidate positions are loaded one step ahead
+            double nq[PS];
+#pragma unroll
+            for (int a = 0; a < PS; ++a) nq[a] = sp[(size_t)rb[s] * PS + a];
+            for (int q = rb[s]; q < re[s]; ++q) {
+                double cq[PS];
+#pragma unroll
+                for (int a = 0; a < PS; ++a) cq[a] = nq[a];
+                const int qn = q + 1 < re[s] ? q + 1 : q;
+#pragma unroll
+                for (int a = 0; a < PS; ++a) nq[a] = sp[(size_t)qn * PS + a];
+                if (q == p) continue;
+                double b[3];
+                b[0] = minimg(cq[0] - xi, g.L[0], g.invL[0], g.periodic[0]);
+                b[1] = minimg(cq[1] - yi, g.L[1], g.invL[1], g.periodic[1]);
+                b[2] = (D == 3) ? minimg(cq[2] - zi, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                const double rsq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+                if (!(rsq < g.mr2)) continue;
+                const double dj = sd[q];
+                double f0, f1 = 0.0;
+                double ur[3] = {0, 0, 0};
+                if constexpr (std::is_same<ER, double>::value) f0 = force_divr<double>(P, rsq, di, dj);
+                else f0 = (double)force_divr<float>(P, (float)rsq, (float)di, (float)dj);
+                if (ELA) {
+                    const double *sdp = sdisp + (size_t)f * n * 3;
+                    for (int a = 0; a < 3; ++a) ur[a] = sdp[(size_t)q * 3 + a] - sdp[(size_t)p * 3 + a] + b[a];
+                    const double ursq = ur[0] * ur[0] + ur[1] * ur[1] + ur[2] * ur[2];
+                    if constexpr (std::is_same<ER, double>::value) f1 = force_divr<double>(P, ursq, di, dj);
+                    else f1 = (double)force_divr<float>(P, (float)ursq, (float)di, (float)dj);
+                    if (f0 == 0.0 && f1 == 0.0) continue;
+                } else if (f0 == 0.0) {
+                    continue;
+                }
+                if (live && cnt < cap) {
+                    ER *e = out + (size_t)cnt * ES;
+                    for (int a = 0; a < D; ++a) e[a] = (ER)(b[a] * g.inv_rc);
+                    e[D] = (ER)(rsq * g.inv_rc2);
+                    e[D + 1] = VIR ? (ER)(h * f0 / g.inv_rc2) : (ER)0;
+                    if (ELA) {
+                        ER *t = e + D + 2;
+                        auto T = [&](int a, int c) { return h * (f1 * ur[a] * ur[c] - f0 * b[a] * b[c]); };
+                        for (int a = 0; a < OD; ++a) t[a] = (ER)T(a, a);
+                        if (OD == 2) t[2] = (ER)T(0, 1);
+                        else { t[3] = (ER)T(0, 1); t[4] = (ER)T(0, 2); t[5] = (ER)T(1, 2); }
+                    }
+                }
+                ++cnt;
+            }
+        }
+    }
+    if (live) bcount[(size_t)f * n + p] = cnt;
+    int m = live ? cnt : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxcount, m);
+}
+
+}  // namespace pgt
+
+#include "ik_gather.hpp"

@@ -489,7 +489,7 @@ struct IrvingKirkwood : Calculator {
     pgt_ik_engine *engine = nullptr;
     int engine_dim = 0;
     int precision = PGT_PREC_FP64;
-    int lanes_per_point = 0;
+    int threads_per_block = 0;
     pgt_ik_stats last_stats{};
 
     IrvingKirkwood(std::shared_ptr<ParticleSystem> sys, std::shared_ptr<PairPotential> pot,
@@ -542,7 +542,7 @@ struct IrvingKirkwood : Calculator {
         p.max_rcut = max_rcut;
         check(pgt_ik_create(&p, &engine), "IrvingKirkwood");
         engine_dim = dim;
-        if (lanes_per_point) check(pgt_ik_set_tuning(engine, lanes_per_point, 0), "IrvingKirkwood");
+        if (threads_per_block) check(pgt_ik_set_tuning(engine, threads_per_block, 0), "IrvingKirkwood");
     }
     void compute(const darray &grid)
     {
@@ -762,7 +762,7 @@ PYBIND11_MODULE(_pgt, root)
         .def("printDisplacement", &IrvingKirkwood::printDisplacement)
         // additions beyond the reference surface
         .def("setPrecision", &IrvingKirkwood::setPrecision)
-        .def("setLanesPerPoint", [](IrvingKirkwood &k, int l) { k.lanes_per_point = l; if (k.engine) check(pgt_ik_set_tuning(k.engine, l, 0), "setLanesPerPoint"); })
+        .def("setThreadsPerBlock", [](IrvingKirkwood &k, int b) { k.threads_per_block = b; if (k.engine) check(pgt_ik_set_tuning(k.engine, b, 0), "setThreadsPerBlock"); })
         .def("getStats", &IrvingKirkwood::getStats)
         .def_readonly("max_rcut", &IrvingKirkwood::max_rcut);
     py::class_<CoarseGrainedField, Observable, std::shared_ptr<CoarseGrainedField>>(mi, "CoarseGrainedField");

@@ -222,8 +222,8 @@ class Engine(object):
         _check(lib().pgt_ik_set_stream(self.h, ctypes.c_void_p(stream_handle) if stream_handle else None),
                "pgt_ik_set_stream")
 
-    def set_tuning(self, lanes_per_point=0, frames_per_launch=0):
-        _check(lib().pgt_ik_set_tuning(self.h, lanes_per_point, frames_per_launch), "pgt_ik_set_tuning")
+    def set_tuning(self, threads_per_block=0, frames_per_launch=0):
+        _check(lib().pgt_ik_set_tuning(self.h, threads_per_block, frames_per_launch), "pgt_ik_set_tuning")
 
     def stats(self):
         s = Stats()

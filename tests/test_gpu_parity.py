@@ -20,7 +20,7 @@ TOL = 1e-10
 
 
 def run_both(sysd, grid, fields=("virial",), dim=None, order=5, cg="octic", cg_rcut=2.0, pot="poly12", params=None,
-             scaled_rcut=1.25, max_rcut=None, lpg=None, precision="fp64"):
+             scaled_rcut=1.25, max_rcut=None, block=None, precision="fp64"):
     dim = dim or sysd["dim"]
     params = params or oracle.poly12_params(rcut=scaled_rcut)
     if max_rcut is None:
@@ -28,8 +28,8 @@ def run_both(sysd, grid, fields=("virial",), dim=None, order=5, cg="octic", cg_r
         max_rcut = max(scaled_rcut * d, d)
     e = engine.Engine(dim=dim, pot=pot, params=params, scaled_rcut=scaled_rcut, cg=cg, cg_rcut=cg_rcut, order=order,
                       max_rcut=max_rcut, precision=precision)
-    if lpg:
-        e.set_tuning(lanes_per_point=lpg)
+    if block:
+        e.set_tuning(threads_per_block=block)
     e.set_system(sysd, grid)
     got = e.compute(fields=fields)
     st = e.stats()
@@ -59,10 +59,10 @@ def test_config1_virial_kinetic(gpu):
     assert np.abs(ref["virial"]).max() > 1.0 and np.abs(ref["kinetic"]).max() > 1e-3
 
 
-@pytest.mark.parametrize("lpg", [1, 4, 8])
-def test_lanes_per_point_agree(gpu, lpg):
+@pytest.mark.parametrize("block", [128, 256])
+def test_block_sizes_agree(gpu, block):
     sysd, grid = workload.config_frame(1, 1)
-    got, ref, st = run_both(sysd, grid, fields=("virial", "kinetic"), lpg=lpg)
+    got, ref, st = run_both(sysd, grid, fields=("virial", "kinetic"), block=block)
     assert_parity(got, ref, st, ("virial", "kinetic"))
 
 
