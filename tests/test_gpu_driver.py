@@ -1,0 +1,90 @@
+"""End to end on the GPU: the pyglasstools IK driver sequence (potential,
+coarse-graining function, GSD ingest, ikcalculator, fieldlogger, analyze)
+against the polydisperse12.gsd fixture, through the pybind11 surface; the
+.xyz output is checked against the oracle's fields."""
+import subprocess
+import sys
+import textwrap
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import FIXTURE_GSD, ROOT
+
+pytestmark = pytest.mark.gpu
+
+SCRIPT = textwrap.dedent("""
+    import sys
+    sys.path.insert(0, {root!r})
+    import pyglasstools as pglass
+    import pyglasstools.io as io
+    import pyglasstools.utils as utils
+    import pyglasstools.potential as pot
+    import pyglasstools.irvingkirkwood as ik
+    import pyglasstools.irvingkirkwood.cgfunc as cgfunc
+    poly12 = pot.polydisperse12(v0=1.0, eps=0.2, rcut=1.25)
+    mycgfunc = cgfunc.octic(rcut=2.0, mode="fixedpoint", order=5)
+    mysystem = utils.read_gsd(filename={gsd!r}, checkpointfile="restart.log")
+    MyCalculator = ik.ikcalculator(cgfunc=mycgfunc, dx=0.25)
+    names = ['virialstress_00', 'virialstress_11', 'virialstress_01']
+    mylogger = io.fieldlogger(keyword="test-dump", names=names, solver=MyCalculator)
+    pglass.analyze(frame_list=range(180, 183))
+    print("STATS", MyCalculator.calculator.getStats())
+""")
+
+
+def test_driver_sequence_writes_oracle_fields(gpu, tmp_path):
+    (tmp_path / "drive.py").write_text(SCRIPT.format(root=ROOT, gsd=FIXTURE_GSD))
+    r = subprocess.run([sys.executable, "drive.py"], cwd=str(tmp_path), capture_output=True, text=True, timeout=600)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    from pyglasstools_amd.gsd_reader import open as gsd_open
+    from pyglasstools_amd.irvingkirkwood import cartesian_grid
+
+    lines = open(tmp_path / "test-dump.xyz").read().splitlines()
+    traj = gsd_open(FIXTURE_GSD)
+    f0 = traj[180]
+    L = float(f0.configuration.box[0])
+    grid = cartesian_grid(L, 0.25)
+    d0 = float(f0.particles.diameter.astype(np.float64).max())
+    max_rcut = max(1.25 * d0, d0)
+    G = len(grid)
+    k = 0
+    for frame in (180, 181, 182):
+        assert lines[k] == "%d " % G and lines[k + 1] == "#Frame %d  " % frame
+        fr = traj[frame]
+        # data_gsd.update keeps the first frame's box (utils.py:88-99)
+        sysd = {"dim": 2, "L": np.array([L, L, 1.0]), "periodic": np.array([1, 1, 0]),
+                "pos": fr.particles.position.astype(np.float64), "diam": fr.particles.diameter.astype(np.float64),
+                "mass": fr.particles.mass.astype(np.float64), "vel": fr.particles.velocity.astype(np.float64)}
+        sub = np.arange(0, G, 97)
+        ref = oracle.ik_compute(sysd, grid[sub], max_rcut=max_rcut)["virial"]
+        for n, g in enumerate(sub):
+            cols = lines[k + 2 + g].split(" ")
+            assert cols[0] == "1"
+            assert float(cols[1]) == grid[g, 0] and float(cols[2]) == grid[g, 1]
+            vals = [float(c) for c in cols[3:6]]
+            want = [ref[n, 0, 0], ref[n, 1, 1], ref[n, 1, 0]]
+            for v, w in zip(vals, want):
+                # 6 significant digits in the file
+                assert v == pytest.approx(float("%g" % w), rel=2e-6, abs=1e-300) or abs(v - w) <= 5e-6 * abs(w)
+        k += 2 + G
+    assert open(tmp_path / "restart.log").read() == "Frame 183"
+    traj.close()
+
+
+def test_pybind_scalar_methods(gpu):
+    import pyglasstools_amd as pg
+
+    p = pg.potential.polydisperse12()
+    assert p.get_pairforce([1.0, 0.0], 1.0, 1.0) == pytest.approx(10.142265153709670, rel=1e-14)
+    assert p.cpppairpotential.getRcut(np.array([1.0, 0, 0]), 1.0, 1.0) == pytest.approx(1.5625)
+    c = pg.irvingkirkwood.cgfunc.octic(rcut=2.0, mode="fixedpoint", order=5)
+    x, ri, rij = np.array([0.1, 0.2, 0.0]), np.array([1.0, -0.5, 0.0]), np.array([0.4, 0.9, 0.0])
+    assert c.get_deltafunc(x, ri) == pytest.approx(oracle.cg_delta("octic", 2.0, float(((x - ri) ** 2).sum())),
+                                                   rel=1e-14)
+    assert c.get_bondfunc(x, ri, rij) == pytest.approx(oracle.bond_fixed("octic", 2.0, 5, x, ri, rij), rel=1e-13)
+    s = 0.3
+    assert c.get_objfunc(s, x, ri, rij) == pytest.approx(
+        oracle.cg_delta("octic", 2.0, float(((x - (ri + s * rij)) ** 2).sum())), rel=1e-13)

@@ -1,0 +1,218 @@
+"""Field-level pins of the oracle (IrvingKirkwood::compute restated,
+pyglasstools/irvingkirkwood/IrvingKirkwood.h:74-110).
+
+  * an independent numpy restatement on tiny systems (its own minimum image,
+    numpy Gauss-Legendre nodes, the Octic kernel written as (1 - t^2)^2 and
+    the force from a sympy derivative of the potential)
+  * two-particle closed forms for the reference's selection rule: weight 1
+    with both endpoints in the ball, 1/2 with one, 0 for a bond that only
+    passes through the ball
+  * invariances: 90 degree rotation, periodic translation, particle
+    permutation, cell list == brute force, faithful loop == clean loop
+"""
+import math
+
+import numpy as np
+import pytest
+import sympy as sp
+
+import oracle
+from pyglasstools_amd import workload
+
+RC = 2.0
+P12 = oracle.poly12_params()
+
+
+def _force_fn():
+    r, s = sp.symbols("r s", positive=True)
+    x = (r / s) ** 2
+    v0, c0, c1, c2 = [sp.Float(v, 30) for v in (P12[0], P12[2], P12[3], P12[4])]
+    phi = v0 * (s / r) ** 12 + c0 + c1 * x + c2 * x ** 2
+    return sp.lambdify((r, s), -sp.diff(phi, r) / r, "numpy")
+
+
+FDR = _force_fn()
+
+
+def numpy_field(sysd, grid, rc=RC, order=5, max_rcut=None, eps=0.2, scaled_rcut=1.25):
+    pos = np.asarray(sysd["pos"], float)
+    L = np.asarray(sysd["L"], float)
+    per = np.asarray(sysd["periodic"], bool)
+    diam = np.asarray(sysd["diam"], float)
+    D = sysd["dim"]
+    if max_rcut is None:
+        max_rcut = max(scaled_rcut * diam.max(), diam.max())
+
+    def mi(d):
+        d = d.copy()
+        d[..., per] -= L[per] * np.round(d[..., per] / L[per])
+        return d
+
+    z, w = np.polynomial.legendre.leggauss(order)
+    s, ws = 0.5 * (z + 1), 0.5 * w
+    norm = 15 / (8 * math.pi * rc * rc)
+    T = np.zeros((len(grid), D, D))
+    for g, x in enumerate(np.asarray(grid, float)):
+        dr = mi(x - pos)
+        for i in np.nonzero((dr ** 2).sum(1) < rc * rc)[0]:
+            ri = x - dr[i]
+            rij = mi(pos - pos[i])
+            r2 = (rij ** 2).sum(1)
+            sel = (np.arange(len(pos)) != i) & (r2 < max_rcut ** 2)
+            for j in np.nonzero(sel)[0]:
+                sig = 0.5 * (diam[i] + diam[j]) * (1 - eps * abs(diam[i] - diam[j]))
+                r = math.sqrt(r2[j])
+                f = FDR(r, sig) if r < scaled_rcut * sig else 0.0
+                pts = ri[None, :] + s[:, None] * rij[j][None, :]
+                t = ((x - pts) ** 2).sum(1) / (rc * rc)
+                bond = float(ws @ np.where(t < 1, norm * (1 - t * t) ** 2, 0.0))
+                T[g] += 0.5 * bond * f * np.outer(rij[j][:D], rij[j][:D]).T
+    return T
+
+
+def small_system(n=40, seed=3, dim=2):
+    return workload.make_system(n, dim, seed)
+
+
+def test_independent_numpy_restatement_2d():
+    sysd = small_system(50, 1)
+    grid = workload.cartesian_grid_2d(sysd["L"][0], 5)
+    ref = numpy_field(sysd, grid)
+    got = oracle.ik_compute(sysd, grid)["virial"]
+    scale = np.abs(ref).max()
+    assert scale > 1.0
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-11 * scale)
+
+
+def test_independent_numpy_restatement_3d():
+    sysd = small_system(130, 2, dim=3)
+    grid = workload.cartesian_grid_3d(sysd["L"][0], 3)
+    ref = numpy_field(sysd, grid)
+    got = oracle.ik_compute(sysd, grid)["virial"]
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+
+
+def _pair_system(x0, x1, L=20.0):
+    pos = np.array([[x0, 0.0, 0.0], [x1, 0.0, 0.0]])
+    return {"dim": 2, "L": np.array([L, L, 1.0]), "periodic": np.array([1, 1, 0]), "pos": pos,
+            "diam": np.array([1.0, 1.0]), "mass": np.ones(2), "vel": np.zeros((2, 3))}
+
+
+def _exact_inside(x, ri, rij, rc=RC):
+    P = np.polynomial.Polynomial
+    a = np.asarray(ri, float) - np.asarray(x, float)
+    rij = np.asarray(rij, float)
+    t = P([a @ a, 2 * a @ rij, rij @ rij]) / rc ** 2
+    poly = (1 - t * t) ** 2
+    return 15 / (8 * math.pi * rc ** 2) * (poly.integ()(1.0) - poly.integ()(0.0))
+
+
+def test_two_particles_both_endpoints_in_ball_weight_one():
+    f = 10.142265153709670  # F/r of poly12 at r = sigma = 1
+    sysd = _pair_system(-0.5, 0.5)
+    T = oracle.ik_compute(sysd, np.array([[0.0, 0.0, 0.0]]))["virial"][0]
+    bond = _exact_inside([0, 0, 0], [-0.5, 0, 0], [1.0, 0, 0])
+    assert T[0, 0] == pytest.approx(f * bond, rel=1e-13)
+    assert T[1, 1] == 0.0 and T[0, 1] == 0.0 and T[1, 0] == 0.0
+
+
+def test_two_particles_one_endpoint_in_ball_weight_half():
+    import json
+    import os
+
+    import mpmath as mp
+
+    from conftest import GOLDEN  # noqa: F401
+
+    f = 10.142265153709670
+    sysd = _pair_system(-0.5, 0.5)
+    x = [1.8, 0.0, 0.0]
+    T = oracle.ik_compute(sysd, np.array([x]))["virial"][0]
+    # GL5 along the segment, 50-digit nodes; the segment leaves the support
+    mp.mp.dps = 50
+    nodes = [mp.findroot(lambda z: mp.legendre(5, z), mp.mpf(g)) for g in np.polynomial.legendre.leggauss(5)[0]]
+    val = mp.mpf(0)
+    for r in nodes:
+        dp = mp.diff(lambda z: mp.legendre(5, z), r)
+        wgt = 2 / ((1 - r * r) * dp * dp)
+        s = (1 + r) / 2
+        d2 = (mp.mpf(x[0]) - (mp.mpf(0.5) - s)) ** 2
+        t = d2 / 4
+        val += wgt / 2 * (15 / (32 * mp.pi) * (1 - t * t) ** 2 if t < 1 else 0)
+    assert T[0, 0] == pytest.approx(0.5 * f * float(val), rel=1e-13)
+
+
+def test_bond_through_ball_with_both_endpoints_outside_is_omitted():
+    sysd = _pair_system(-0.5, 0.5)
+    T = oracle.ik_compute(sysd, np.array([[0.0, 0.0, 0.0]]), cg_rcut=0.3)["virial"][0]
+    assert not T.any()
+    # ... although the kernel integral along the bond is not zero there
+    assert oracle.bond_fixed("octic", 0.3, 5, [0, 0, 0], [-0.5, 0, 0], [1, 0, 0]) > 0
+
+
+def test_kinetic_closed_form():
+    sysd = small_system(30, 9)
+    grid = workload.cartesian_grid_2d(sysd["L"][0], 4)
+    got = oracle.ik_compute(sysd, grid, fields=("kinetic",))["kinetic"]
+    L = sysd["L"][:2]
+    for g, x in enumerate(grid):
+        d = sysd["pos"][:, :2] - x[:2]
+        d -= L * np.round(d / L)
+        t = (d ** 2).sum(1) / RC ** 2
+        wgt = np.where(t < 1, 15 / (8 * math.pi * RC ** 2) * (1 - t * t) ** 2, 0.0)
+        v = sysd["vel"][:, :2]
+        want = np.einsum("i,ia,ib->ba", wgt * sysd["mass"], v, v)
+        np.testing.assert_allclose(got[g], want, rtol=1e-12, atol=1e-15)
+
+
+def test_rotation_by_90_degrees():
+    sysd = small_system(60, 4)
+    grid = workload.cartesian_grid_2d(sysd["L"][0], 6)[:20]
+    rot = dict(sysd)
+    p = sysd["pos"].copy()
+    rot["pos"] = np.stack([-p[:, 1], p[:, 0], p[:, 2]], 1)
+    g2 = np.stack([-grid[:, 1], grid[:, 0], grid[:, 2]], 1)
+    A = oracle.ik_compute(sysd, grid)["virial"]
+    B = oracle.ik_compute(rot, g2)["virial"]
+    tol = 1e-12 * np.abs(A).max()
+    np.testing.assert_allclose(B[:, 0, 0], A[:, 1, 1], atol=tol, rtol=0)
+    np.testing.assert_allclose(B[:, 1, 1], A[:, 0, 0], atol=tol, rtol=0)
+    np.testing.assert_allclose(B[:, 0, 1], -A[:, 0, 1], atol=tol, rtol=0)
+
+
+def test_periodic_translation():
+    sysd = small_system(60, 5)
+    grid = workload.cartesian_grid_2d(sysd["L"][0], 6)
+    sh = np.array([0.37, -1.21, 0.0])
+    L = sysd["L"]
+    moved = dict(sysd)
+    q = sysd["pos"] + sh
+    q[:, :2] -= L[:2] * np.floor((q[:, :2] + L[:2] / 2) / L[:2])
+    moved["pos"] = q
+    A = oracle.ik_compute(sysd, grid)["virial"]
+    B = oracle.ik_compute(moved, grid + sh)["virial"]
+    np.testing.assert_allclose(B, A, atol=1e-12 * np.abs(A).max(), rtol=0)
+
+
+def test_particle_permutation():
+    sysd = small_system(60, 6)
+    grid = workload.cartesian_grid_2d(sysd["L"][0], 6)
+    perm = np.random.default_rng(1).permutation(60)
+    shuffled = {k: (v[perm] if isinstance(v, np.ndarray) and v.shape[:1] == (60,) else v) for k, v in sysd.items()}
+    A = oracle.ik_compute(sysd, grid, fields=("virial", "kinetic"))
+    B = oracle.ik_compute(shuffled, grid, fields=("virial", "kinetic"))
+    for k in ("virial", "kinetic"):
+        np.testing.assert_allclose(B[k], A[k], atol=1e-12 * np.abs(A[k]).max(), rtol=0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_cell_list_equals_brute_force_and_loops_agree(dim):
+    sysd = small_system(400 if dim == 2 else 700, 7, dim)
+    grid = (workload.cartesian_grid_2d(sysd["L"][0], 8) if dim == 2 else workload.cartesian_grid_3d(sysd["L"][0], 4))
+    a = oracle.ik_compute(sysd, grid, fields=("virial", "kinetic"))
+    b = oracle.ik_compute(sysd, grid, fields=("virial", "kinetic"), brute=True)
+    c = oracle.ik_compute(sysd, grid, fields=("virial", "kinetic"), mode=0)
+    for k in ("virial", "kinetic"):
+        np.testing.assert_allclose(b[k], a[k], atol=1e-12 * np.abs(a[k]).max(), rtol=0)
+        assert np.array_equal(c[k], a[k])
+    assert a["n_triples"] == b["n_triples"] and a["n_nonzero"] == b["n_nonzero"] and a["n_ball"] == b["n_ball"]
