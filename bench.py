@@ -101,6 +101,7 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity sample")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-buffer API timing (for profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--block", type=int, default=0, help="gather threads per workgroup (0: autotune)")
     ap.add_argument("--fpl", type=int, default=0, help="frames per launch (0: autotune)")
@@ -204,6 +205,23 @@ def main():
              "gather_share_of_step": gather_ms / 1e3 / elapsed, "tuning": tuned, "precision": args.precision}
     if trials:
         extra["autotune_trials"] = trials
+
+    # host-buffer API (what IrvingKirkwood.compute uses): per-frame upload,
+    # compute and download through pgt_ik_compute, PCIe copies included
+    if not args.no_host:
+        host = engine.Engine(dim=2, max_rcut=max_rcut, precision=args.precision)
+        host.set_box(L, [1, 1, 0])
+        host.set_grid(grid)
+        nh = min(F, 10)
+        for f in range(2):
+            host.set_particles(systems[f]["pos"], systems[f]["diam"], systems[f]["mass"], systems[f]["vel"])
+            host.compute(fields=("virial", "kinetic"))
+        t0 = time.perf_counter()
+        for f in range(nh):
+            host.set_particles(systems[f]["pos"], systems[f]["diam"], systems[f]["mass"], systems[f]["vel"])
+            host.compute(fields=("virial", "kinetic"))
+        extra["host_api_fps_pcie_inclusive"] = nh / (time.perf_counter() - t0)
+        host.close()
 
     cpu = None
     if not args.no_cpu:

@@ -381,16 +381,19 @@ void launch_bonds_t(const LaunchCtx &c, unsigned fields, const PotParams &P, con
                     int *bcount, void *bonds, int *maxcount)
 {
     const bool V = fields & PGT_FIELD_VIRIAL, E = fields & PGT_FIELD_ELASTIC;
-    dim3 grid((unsigned)((c.n + 255) / 256), (unsigned)c.nf);
+    // candidate list per lane: ~2.5x the mean number of neighbours within
+    // max_rcut at unit density (13 in 2D, 35 in 3D for max_rcut ~ 2)
+    constexpr int BLK = 128, MAXC = D == 2 ? 32 : 64;
+    dim3 grid((unsigned)((c.n + BLK - 1) / BLK), (unsigned)c.nf);
     if (V && E)
-        hipLaunchKernelGGL((k_bonds<D, OD, true, true, ER>), grid, dim3(256), 0, c.stream, c.g, c.n, P, c.cgnorm,
-                           c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, maxcount);
+        hipLaunchKernelGGL((k_bonds<D, OD, true, true, ER, BLK, MAXC>), grid, dim3(BLK), 0, c.stream, c.g, c.n, P,
+                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, maxcount);
     else if (E)
-        hipLaunchKernelGGL((k_bonds<D, OD, false, true, ER>), grid, dim3(256), 0, c.stream, c.g, c.n, P, c.cgnorm,
-                           c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, maxcount);
+        hipLaunchKernelGGL((k_bonds<D, OD, false, true, ER, BLK, MAXC>), grid, dim3(BLK), 0, c.stream, c.g, c.n, P,
+                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, maxcount);
     else
-        hipLaunchKernelGGL((k_bonds<D, OD, true, false, ER>), grid, dim3(256), 0, c.stream, c.g, c.n, P, c.cgnorm,
-                           c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, maxcount);
+        hipLaunchKernelGGL((k_bonds<D, OD, true, false, ER, BLK, MAXC>), grid, dim3(BLK), 0, c.stream, c.g, c.n, P,
+                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, maxcount);
 }
 
 template <int D, int OD, typename ER>

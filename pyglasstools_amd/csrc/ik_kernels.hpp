@@ -2,8 +2,8 @@
 // Included once, by ik_engine.hip.
 //
 //   cell list   k_cell_count -> k_cell_scan -> k_cell_fill -> k_cell_rank
-//   bond list   k_bonds
-//   fields      k_gather
+//   bond list   k_bonds   (ik_bonds.hpp)
+//   fields      k_gather  (ik_gather.hpp)
 //
 // All per-frame arrays are stacked along blockIdx.y (one frame per y index).
 #pragma once
@@ -202,115 +202,7 @@ __global__ void k_cell_rank(int n, int ncell, const int *__restrict__ pcell, con
         for (int a = 0; a < 3; ++a) sdisp[fk * 3 + a] = disp ? disp[fi * 3 + a] : 0.0;
 }
 
-// ------------------------------------------------------------------------
-// bond list
-// ------------------------------------------------------------------------
-// Entries of particle p (cell-sorted index) sit at bonds[(f*n + p)*cap*ES ...],
-// one per neighbour j within max_rcut with a nonzero pair force (or, with the
-// elastic field, a nonzero force at r_ij + u_ij).  Layout, in units of ER:
-//   [0, D)     b' = r_ij / rc                   (scaled bond vector)
-//   [D]        C' = |r_ij|^2 / rc^2
-//   [D+1]      c' = 0.5 * N_cg * (F/r) * rc^2   (virial: T += c' w b' (x) b')
-//   [D+2, ..)  elastic: 0.5 * N_cg * (F1_a (u+r)_b - F_a r_b), packed symmetric
-// padded to a 16-byte multiple.  N_cg is the kernel normalisation and w the
-// quadrature sum of the unnormalised kernel shape.
-template <int D, int OD, bool ELA, typename ER>
-struct BondLayout {
-    static constexpr int KN = OD * (OD + 1) / 2;
-    static constexpr int RAW = D + 2 + (ELA ? KN : 0);
-    static constexpr int PER16 = 16 / (int)sizeof(ER);
-    static constexpr int ES = ((RAW + PER16 - 1) / PER16) * PER16;
-};
-
-template <int D, int OD, bool VIR, bool ELA, typename ER>
-__global__ __launch_bounds__(256) void k_bonds(Geom g, int n, PotParams P, double cgnorm,
-                                               const int *__restrict__ cstart, const double *__restrict__ spos,
-                                               const double *__restrict__ sdiam, const double *__restrict__ sdisp,
-                                               int cap, int *__restrict__ bcount, ER *__restrict__ bonds,
-                                               int *__restrict__ maxcount)
-{
-    constexpr int PS = (D == 2) ? 2 : 4;
-    using LY = BondLayout<D, OD, ELA, ER>;
-    constexpr int ES = LY::ES;
-    const int f = blockIdx.y;
-    const int p_raw = blockIdx.x * blockDim.x + threadIdx.x;
-    // lanes past the end shadow the last particle without writing, so the
-    // whole wavefront takes part in the max-reduction at the end
-    const bool live = p_raw < n;
-    const int p = live ? p_raw : n - 1;
-    const double *sp = spos + (size_t)f * n * PS;
-    const double *sd = sdiam + (size_t)f * n;
-    const int *st = cstart + (size_t)f * (g.ncell + 1);
-    const double xi = sp[(size_t)p * PS], yi = sp[(size_t)p * PS + 1], zi = (D == 3) ? sp[(size_t)p * PS + 2] : 0.0;
-    const double di = sd[p];
-    const int cx = cell_coord(g, 0, xi), cy = cell_coord(g, 1, yi), cz = cell_coord(g, 2, zi);
-    int rows[9];
-    const int nrows = neighbour_rows(g, cy, cz, rows);
-    ER *out = bonds + ((size_t)f * n + p) * (size_t)cap * ES;
-    const double h = 0.5 * cgnorm;
-    int cnt = 0;
-    for (int r = 0; r < nrows; ++r) {
-        int rb[2], re[2];
-        const int nr = row_ranges(g, st, rows[r], cx, rb, re);
-        for (int s = 0; s < nr; ++s) {
-            if (rb[s] >= re[s]) continue;
-            // candidate positions are loaded one step ahead
-            double nq[PS];
-#pragma unroll
-            for (int a = 0; a < PS; ++a) nq[a] = sp[(size_t)rb[s] * PS + a];
-            for (int q = rb[s]; q < re[s]; ++q) {
-                double cq[PS];
-#pragma unroll
-                for (int a = 0; a < PS; ++a) cq[a] = nq[a];
-                const int qn = q + 1 < re[s] ? q + 1 : q;
-#pragma unroll
-                for (int a = 0; a < PS; ++a) nq[a] = sp[(size_t)qn * PS + a];
-                if (q == p) continue;
-                double b[3];
-                b[0] = minimg(cq[0] - xi, g.L[0], g.invL[0], g.periodic[0]);
-                b[1] = minimg(cq[1] - yi, g.L[1], g.invL[1], g.periodic[1]);
-                b[2] = (D == 3) ? minimg(cq[2] - zi, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
-                const double rsq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
-                if (!(rsq < g.mr2)) continue;
-                const double dj = sd[q];
-                double f0, f1 = 0.0;
-                double ur[3] = {0, 0, 0};
-                if constexpr (std::is_same<ER, double>::value) f0 = force_divr<double>(P, rsq, di, dj);
-                else f0 = (double)force_divr<float>(P, (float)rsq, (float)di, (float)dj);
-                if (ELA) {
-                    const double *sdp = sdisp + (size_t)f * n * 3;
-                    for (int a = 0; a < 3; ++a) ur[a] = sdp[(size_t)q * 3 + a] - sdp[(size_t)p * 3 + a] + b[a];
-                    const double ursq = ur[0] * ur[0] + ur[1] * ur[1] + ur[2] * ur[2];
-                    if constexpr (std::is_same<ER, double>::value) f1 = force_divr<double>(P, ursq, di, dj);
-                    else f1 = (double)force_divr<float>(P, (float)ursq, (float)di, (float)dj);
-                    if (f0 == 0.0 && f1 == 0.0) continue;
-                } else if (f0 == 0.0) {
-                    continue;
-                }
-                if (live && cnt < cap) {
-                    ER *e = out + (size_t)cnt * ES;
-                    for (int a = 0; a < D; ++a) e[a] = (ER)(b[a] * g.inv_rc);
-                    e[D] = (ER)(rsq * g.inv_rc2);
-                    e[D + 1] = VIR ? (ER)(h * f0 / g.inv_rc2) : (ER)0;
-                    if (ELA) {
-                        ER *t = e + D + 2;
-                        auto T = [&](int a, int c) { return h * (f1 * ur[a] * ur[c] - f0 * b[a] * b[c]); };
-                        for (int a = 0; a < OD; ++a) t[a] = (ER)T(a, a);
-                        if (OD == 2) t[2] = (ER)T(0, 1);
-                        else { t[3] = (ER)T(0, 1); t[4] = (ER)T(0, 2); t[5] = (ER)T(1, 2); }
-                    }
-                }
-                ++cnt;
-            }
-        }
-    }
-    if (live) bcount[(size_t)f * n + p] = cnt;
-    int m = live ? cnt : 0;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(maxcount, m);
-}
-
 }  // namespace pgt
 
+#include "ik_bonds.hpp"
 #include "ik_gather.hpp"
