@@ -27,7 +27,7 @@ struct BondLayout {
 };
 
 // One lane per particle.  Pass 1 walks the 3x3(x3) cells around the particle
-// with the next candidate's position loaded ahead and keeps, in a per-lane
+// (candidate positions loaded a batch at a time) and keeps, in a per-lane
 // LDS list, the candidates within max_rcut (about a third of them).  Pass 2
 // evaluates the pair force only for listed candidates, so the force code runs
 // with the lanes of a wavefront in step instead of behind a branch that some
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
                                                const int *__restrict__ cstart, const double *__restrict__ spos,
                                                const double *__restrict__ sdiam, const double *__restrict__ sdisp,
                                                int cap, int *__restrict__ bcount, ER *__restrict__ bonds,
-                                               int *__restrict__ maxcount)
+                                               int *__restrict__ wave_max)
 {
     constexpr int PS = (D == 2) ? 2 : 4;
     using LY = BondLayout<D, OD, ELA, ER>;
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
     __shared__ int lq[MAXC][BLK];
     const int lane = threadIdx.x;
     const int f = blockIdx.y;
-    const int p_raw = blockIdx.x * BLK + lane;
+    const int p_raw = xcd_tile(blockIdx.x, gridDim.x) * BLK + lane;
     // lanes past the end shadow the last particle without writing, so the
     // whole wavefront takes part in the max-reduction at the end
     const bool live = p_raw < n;
@@ -61,48 +61,65 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
     const double h = 0.5 * cgnorm;
     int cnt = 0;
 
-    // pass 2 over the first m listed candidates
+    // pass 2 over the first m listed candidates, NF at a time with their
+    // positions and diameters loaded together
+    constexpr int NF = 4;
     auto forces = [&](int m) {
-        for (int k = 0; k < m; ++k) {
-            const int q = lq[k][lane];
-            double b[3];
-            b[0] = minimg(sp[(size_t)q * PS] - xi, g.L[0], g.invL[0], g.periodic[0]);
-            b[1] = minimg(sp[(size_t)q * PS + 1] - yi, g.L[1], g.invL[1], g.periodic[1]);
-            b[2] = (D == 3) ? minimg(sp[(size_t)q * PS + 2] - zi, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
-            const double rsq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
-            const double dj = sd[q];
-            double f0, f1 = 0.0;
-            double ur[3] = {0, 0, 0};
-            if constexpr (std::is_same<ER, double>::value) f0 = force_divr<double>(P, rsq, di, dj);
-            else f0 = (double)force_divr<float>(P, (float)rsq, (float)di, (float)dj);
-            if (ELA) {
-                const double *sdp = sdisp + (size_t)f * n * 3;
-                for (int a = 0; a < 3; ++a) ur[a] = sdp[(size_t)q * 3 + a] - sdp[(size_t)p * 3 + a] + b[a];
-                const double ursq = ur[0] * ur[0] + ur[1] * ur[1] + ur[2] * ur[2];
-                if constexpr (std::is_same<ER, double>::value) f1 = force_divr<double>(P, ursq, di, dj);
-                else f1 = (double)force_divr<float>(P, (float)ursq, (float)di, (float)dj);
-                if (f0 == 0.0 && f1 == 0.0) continue;
-            } else if (f0 == 0.0) {
-                continue;
+        for (int k0 = 0; k0 < m; k0 += NF) {
+            int qq[NF];
+            double cqs[NF][PS], djs[NF];
+#pragma unroll
+            for (int u = 0; u < NF; ++u) {
+                qq[u] = lq[min(k0 + u, m - 1)][lane];
+#pragma unroll
+                for (int a = 0; a < PS; ++a) cqs[u][a] = sp[(size_t)qq[u] * PS + a];
+                djs[u] = sd[qq[u]];
             }
-            if (live && cnt < cap) {
-                ER *e = out + (size_t)cnt * ES;
-                for (int a = 0; a < D; ++a) e[a] = (ER)(b[a] * g.inv_rc);
-                e[D] = (ER)(rsq * g.inv_rc2);
-                e[D + 1] = VIR ? (ER)(h * f0 / g.inv_rc2) : (ER)0;
+#pragma unroll
+            for (int u = 0; u < NF; ++u) {
+                if (k0 + u >= m) break;
+                const int q = qq[u];
+                const double dj = djs[u];
+                double b[3];
+                b[0] = minimg(cqs[u][0] - xi, g.L[0], g.invL[0], g.periodic[0]);
+                b[1] = minimg(cqs[u][1] - yi, g.L[1], g.invL[1], g.periodic[1]);
+                b[2] = (D == 3) ? minimg(cqs[u][2] - zi, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                const double rsq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+                double f0, f1 = 0.0;
+                double ur[3] = {0, 0, 0};
+                if constexpr (std::is_same<ER, double>::value) f0 = force_divr<double>(P, rsq, di, dj);
+                else f0 = (double)force_divr<float>(P, (float)rsq, (float)di, (float)dj);
                 if (ELA) {
-                    ER *t = e + D + 2;
-                    auto T = [&](int a, int c) { return h * (f1 * ur[a] * ur[c] - f0 * b[a] * b[c]); };
-                    for (int a = 0; a < OD; ++a) t[a] = (ER)T(a, a);
-                    if (OD == 2) t[2] = (ER)T(0, 1);
-                    else { t[3] = (ER)T(0, 1); t[4] = (ER)T(0, 2); t[5] = (ER)T(1, 2); }
+                    const double *sdp = sdisp + (size_t)f * n * 3;
+                    for (int a = 0; a < 3; ++a) ur[a] = sdp[(size_t)q * 3 + a] - sdp[(size_t)p * 3 + a] + b[a];
+                    const double ursq = ur[0] * ur[0] + ur[1] * ur[1] + ur[2] * ur[2];
+                    if constexpr (std::is_same<ER, double>::value) f1 = force_divr<double>(P, ursq, di, dj);
+                    else f1 = (double)force_divr<float>(P, (float)ursq, (float)di, (float)dj);
+                    if (f0 == 0.0 && f1 == 0.0) continue;
+                } else if (f0 == 0.0) {
+                    continue;
                 }
+                if (live && cnt < cap) {
+                    ER *e = out + (size_t)cnt * ES;
+                    for (int a = 0; a < D; ++a) e[a] = (ER)(b[a] * g.inv_rc);
+                    e[D] = (ER)(rsq * g.inv_rc2);
+                    e[D + 1] = VIR ? (ER)(h * f0 / g.inv_rc2) : (ER)0;
+                    if (ELA) {
+                        ER *t = e + D + 2;
+                        auto T = [&](int a, int c) { return h * (f1 * ur[a] * ur[c] - f0 * b[a] * b[c]); };
+                        for (int a = 0; a < OD; ++a) t[a] = (ER)T(a, a);
+                        if (OD == 2) t[2] = (ER)T(0, 1);
+                        else { t[3] = (ER)T(0, 1); t[4] = (ER)T(0, 2); t[5] = (ER)T(1, 2); }
+                    }
+                }
+                ++cnt;
             }
-            ++cnt;
         }
     };
 
-    // pass 1
+    // pass 1: candidates in batches of NB, the batch's loads issued together
+    constexpr int NB = 8;
+    static_assert(MAXC >= NB, "list shorter than a batch");
     const int cx = cell_coord(g, 0, xi), cy = cell_coord(g, 1, yi), cz = cell_coord(g, 2, zi);
     int rows[9];
     const int nrows = neighbour_rows(g, cy, cz, rows);
@@ -111,26 +128,28 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
         int rb[2], re[2];
         const int nr = row_ranges(g, st, rows[r], cx, rb, re);
         for (int s = 0; s < nr; ++s) {
-            if (rb[s] >= re[s]) continue;
-            double nq[PS];
-#pragma unroll
-            for (int a = 0; a < PS; ++a) nq[a] = sp[(size_t)rb[s] * PS + a];
-            for (int q = rb[s]; q < re[s]; ++q) {
-                double cq[PS];
-#pragma unroll
-                for (int a = 0; a < PS; ++a) cq[a] = nq[a];
-                const int qn = q + 1 < re[s] ? q + 1 : q;
-#pragma unroll
-                for (int a = 0; a < PS; ++a) nq[a] = sp[(size_t)qn * PS + a];
-                const double b0 = minimg(cq[0] - xi, g.L[0], g.invL[0], g.periodic[0]);
-                const double b1 = minimg(cq[1] - yi, g.L[1], g.invL[1], g.periodic[1]);
-                const double b2 = (D == 3) ? minimg(cq[2] - zi, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
-                const double rsq = b0 * b0 + b1 * b1 + b2 * b2;
-                if (q == p || !(rsq < g.mr2)) continue;
-                lq[m][lane] = q;
-                if (++m == MAXC) {
+            for (int q0 = rb[s]; q0 < re[s]; q0 += NB) {
+                if (m > MAXC - NB) {  // room for a whole batch
                     forces(m);
                     m = 0;
+                }
+                double cq[NB][PS];
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    const int q = min(q0 + u, re[s] - 1);
+#pragma unroll
+                    for (int a = 0; a < PS; ++a) cq[u][a] = sp[(size_t)q * PS + a];
+                }
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    const int q = q0 + u;
+                    if (q >= re[s]) break;
+                    const double b0 = minimg(cq[u][0] - xi, g.L[0], g.invL[0], g.periodic[0]);
+                    const double b1 = minimg(cq[u][1] - yi, g.L[1], g.invL[1], g.periodic[1]);
+                    const double b2 = (D == 3) ? minimg(cq[u][2] - zi, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                    const double rsq = b0 * b0 + b1 * b1 + b2 * b2;
+                    if (q == p || !(rsq < g.mr2)) continue;
+                    lq[m++][lane] = q;
                 }
             }
         }
@@ -141,7 +160,9 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
     int mc = live ? cnt : 0;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) mc = max(mc, __shfl_xor(mc, off, 64));
-    if ((lane & 63) == 0) atomicMax(maxcount, mc);
+    // the largest count of the wavefront, stored per wavefront in launch
+    // order (k_reduce_stats takes the maximum)
+    if ((lane & 63) == 0) wave_max[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (BLK / 64) + (lane >> 6)] = mc;
 }
 
 }  // namespace pgt

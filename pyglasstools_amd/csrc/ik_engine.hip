@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -134,9 +135,13 @@ int gl_rule(int n, std::vector<double> &s, std::vector<double> &w)
     return 0;
 }
 
-// Morton (Z-order) permutation of the grid points on a lattice of spacing
-// ~ (extent^d / G)^(1/d), so that the 64 lanes of a wavefront take a compact
-// block of grid points and share the particles they read.
+// Morton (Z-order) permutation of the grid points, so that a wavefront or a
+// workgroup takes a compact block of grid points and shares the particles it
+// reads.  Points are rounded onto a lattice whose spacing along each axis is
+// extent / (n - 1), n = G^(1/d): a regular n^d grid whose end points sit on
+// the box faces (as numpy.linspace makes them) maps onto the indices 0..n-1
+// exactly, so for n a power of two every run of 4^k (2D) or 8^k (3D) points
+// is a square (cube) block.
 std::vector<int> morton_order(int64_t G, const double *grid, bool use_z)
 {
     std::vector<int> perm((size_t)G);
@@ -149,16 +154,17 @@ std::vector<int> morton_order(int64_t G, const double *grid, bool use_z)
             hi[a] = std::max(hi[a], grid[3 * i + a]);
         }
     const int d = use_z ? 3 : 2;
-    double vol = 1.0;
-    for (int a = 0; a < d; ++a) vol *= std::max(hi[a] - lo[a], 1e-12);
-    const double h = std::pow(vol / (double)G, 1.0 / d);
+    const double nside = std::max(2.0, std::round(std::pow((double)G, 1.0 / d)));
+    double h[3];
+    for (int a = 0; a < 3; ++a) h[a] = std::max(hi[a] - lo[a], 1e-300) / (nside - 1.0);
     const int bits = use_z ? 10 : 16;
     const double qmax = (double)((1u << bits) - 1);
     std::vector<uint64_t> key((size_t)G);
     for (int64_t i = 0; i < G; ++i) {
         uint64_t k = 0;
         uint32_t q[3] = {0, 0, 0};
-        for (int a = 0; a < d; ++a) q[a] = (uint32_t)std::min(qmax, std::max(0.0, std::floor((grid[3 * i + a] - lo[a]) / h)));
+        for (int a = 0; a < d; ++a)
+            q[a] = (uint32_t)std::min(qmax, std::max(0.0, std::floor((grid[3 * i + a] - lo[a]) / h[a] + 0.5)));
         for (int b = bits - 1; b >= 0; --b)
             for (int a = d - 1; a >= 0; --a) k = (k << 1) | ((q[a] >> b) & 1u);
         key[(size_t)i] = k;
@@ -217,7 +223,8 @@ struct pgt_ik_engine {
     DBuf<unsigned char> d_bonds;
     DBuf<double> d_out;
     DBuf<double> d_gl;
-    DBuf<unsigned long long> d_stats;
+    DBuf<unsigned long long> d_stats, d_wcounts;  // job totals; per-wavefront counts of a batch
+    DBuf<int> d_wmax;                             // per-wavefront largest bond count of a batch
     std::vector<double> gl_s, gl_w;
     int cap = 0;
     int block = 256;
@@ -376,24 +383,26 @@ void launch_gather_c(const LaunchCtx &c, unsigned fields, int cg)
     }
 }
 
+constexpr int kBondBlock = 128;  // k_bonds threads per workgroup
+
 template <int D, int OD, typename ER>
 void launch_bonds_t(const LaunchCtx &c, unsigned fields, const PotParams &P, const double *sdiam, const double *sdisp,
-                    int *bcount, void *bonds, int *maxcount)
+                    int *bcount, void *bonds, int *wave_max)
 {
     const bool V = fields & PGT_FIELD_VIRIAL, E = fields & PGT_FIELD_ELASTIC;
     // candidate list per lane: ~2.5x the mean number of neighbours within
     // max_rcut at unit density (13 in 2D, 35 in 3D for max_rcut ~ 2)
-    constexpr int BLK = 128, MAXC = D == 2 ? 32 : 64;
+    constexpr int BLK = kBondBlock, MAXC = D == 2 ? 32 : 64;
     dim3 grid((unsigned)((c.n + BLK - 1) / BLK), (unsigned)c.nf);
     if (V && E)
         hipLaunchKernelGGL((k_bonds<D, OD, true, true, ER, BLK, MAXC>), grid, dim3(BLK), 0, c.stream, c.g, c.n, P,
-                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, maxcount);
+                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, wave_max);
     else if (E)
         hipLaunchKernelGGL((k_bonds<D, OD, false, true, ER, BLK, MAXC>), grid, dim3(BLK), 0, c.stream, c.g, c.n, P,
-                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, maxcount);
+                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, wave_max);
     else
         hipLaunchKernelGGL((k_bonds<D, OD, true, false, ER, BLK, MAXC>), grid, dim3(BLK), 0, c.stream, c.g, c.n, P,
-                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, maxcount);
+                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, wave_max);
 }
 
 template <int D, int OD, typename ER>
@@ -436,15 +445,14 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     if (ela && (rc = e->d_sdisp.ensure((size_t)nf * n * 3))) return rc;
     hipStream_t s = e->stream;
     if (ev_begin) HIPCHK(hipEventRecord(ev_begin, s));
+    double *skin = kin ? e->d_skin.p : nullptr;
+    double *sdisp = ela ? e->d_sdisp.p : nullptr;
     HIPCHK(hipMemsetAsync(e->d_ccount.p, 0, sizeof(int) * (size_t)nf * g.ncell, s));
-
     const dim3 pgrid((unsigned)((n + 255) / 256), (unsigned)nf);
     hipLaunchKernelGGL(k_cell_count, pgrid, dim3(256), 0, s, g, n, pos, e->d_pcell.p, e->d_pslot.p, e->d_ccount.p);
     hipLaunchKernelGGL(k_cell_scan, dim3(nf), dim3(1024), 0, s, g.ncell, e->d_ccount.p, e->d_cstart.p);
-    hipLaunchKernelGGL(k_cell_fill, pgrid, dim3(256), 0, s, n, g.ncell, e->d_pcell.p, e->d_pslot.p, e->d_cstart.p,
-                       e->d_order.p);
-    double *skin = kin ? e->d_skin.p : nullptr;
-    double *sdisp = ela ? e->d_sdisp.p : nullptr;
+    hipLaunchKernelGGL(k_cell_fill, pgrid, dim3(256), 0, s, n, g.ncell, e->d_pcell.p, e->d_pslot.p,
+                       e->d_cstart.p, e->d_order.p);
     if (D == 2)
         hipLaunchKernelGGL((k_cell_rank<2, 2>), pgrid, dim3(256), 0, s, n, g.ncell, e->d_pcell.p, e->d_cstart.p,
                            e->d_order.p, pos, diam, mass, vel, disp, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
@@ -477,10 +485,14 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     c.out.vir = ovir;
     c.out.kin = okin;
     c.out.ela = oela;
-    c.out.n_ball = e->d_stats.p;
-    c.out.n_contrib = e->d_stats.p + 1;
     c.stream = s;
 
+    // per-wavefront statistics of this batch (see k_reduce_stats)
+    const int nw_gather = (int)(((size_t)G + c.block - 1) / c.block) * nf * (c.block / 64);
+    const int nw_bonds = pairs ? (int)(((size_t)n + kBondBlock - 1) / kBondBlock) * nf * (kBondBlock / 64) : 0;
+    if ((rc = e->d_wcounts.ensure(2 * (size_t)std::max(nw_gather, 1))) || (rc = e->d_wmax.ensure((size_t)std::max(nw_bonds, 1))))
+        return rc;
+    c.out.wave_counts = e->d_wcounts.p;
     if (pairs) {
         // entries beyond the capacity are counted, not written; the caller
         // checks d_maxcount after the job and re-runs with a larger capacity
@@ -489,13 +501,13 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
         c.cap = e->cap;
         const PotParams P = make_pot(e->prm);
         if (D == 2) {
-            if (fp32) launch_bonds_t<2, 2, float>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_maxcount.p);
-            else launch_bonds_t<2, 2, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_maxcount.p);
+            if (fp32) launch_bonds_t<2, 2, float>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p);
+            else launch_bonds_t<2, 2, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p);
         } else if (OD == 2) {
-            launch_bonds_t<3, 2, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_maxcount.p);
+            launch_bonds_t<3, 2, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p);
         } else {
-            if (fp32) launch_bonds_t<3, 3, float>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_maxcount.p);
-            else launch_bonds_t<3, 3, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_maxcount.p);
+            if (fp32) launch_bonds_t<3, 3, float>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p);
+            else launch_bonds_t<3, 3, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p);
         }
         HIPCHK(hipGetLastError());
     } else {
@@ -517,6 +529,9 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     }
     HIPCHK(hipGetLastError());
     if (ev_end) HIPCHK(hipEventRecord(ev_end, s));
+    hipLaunchKernelGGL(k_reduce_stats, dim3(1), dim3(1024), 0, s, nw_gather, e->d_wcounts.p, nw_bonds, e->d_wmax.p,
+                       e->d_stats.p, e->d_maxcount.p);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 
@@ -647,6 +662,8 @@ void pgt_ik_destroy(pgt_ik_engine *e)
     for (auto *b : ibufs) b->release();
     e->d_bonds.release();
     e->d_stats.release();
+    e->d_wcounts.release();
+    e->d_wmax.release();
     for (hipEvent_t v : e->ev) (void)hipEventDestroy(v);
     if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
     delete e;

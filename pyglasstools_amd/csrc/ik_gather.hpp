@@ -13,8 +13,11 @@ struct GatherOut {
     double *vir;
     double *kin;
     double *ela;
-    unsigned long long *n_ball;
-    unsigned long long *n_contrib;
+    // per-wavefront counts (balls, contributions), two per wavefront in
+    // launch order; summed by k_reduce_stats.  One device-scope atomic per
+    // wavefront on a shared counter would serialise every wavefront of the
+    // launch on a single address.
+    unsigned long long *wave_counts;
 };
 
 // sum_k w_k shape(t(s_k)), t(s) = A + s B + s^2 C: the bond integral of the
@@ -63,15 +66,16 @@ __device__ __forceinline__ void load_entry(const ER *__restrict__ p, ER (&v)[ES]
 }
 
 // Field gather, one lane per grid point (grid points taken in the order of
-// `perm`, which puts nearby points in the same wavefront).
+// `perm`, which puts nearby points in the same wavefront), reading particles
+// and bond entries straight from global memory.
 //
 // Pass A scans the cells around the grid point and compacts the particles
-// inside the coarse-graining ball, with their bond counts, into a per-lane
-// list in LDS (one int per entry, index | count << 26, when PACK is set:
-// fewer than 2^26 particles and at most 63 bonds each; otherwise a second
-// array holds the counts).  The list is then sorted by bond count (largest first), so
-// that at every step of pass B the lanes of a wavefront hold particles with
-// similar bond counts and the bond loop wastes few lanes.  Pass B adds each
+// inside the coarse-graining ball, with their bond counts, into a per-lane list in LDS
+// (one int per entry, index | count << 26, when PACK is set: fewer than 2^26
+// particles and at most 63 bonds each; otherwise a second array holds the
+// counts).  The list is then sorted by bond count (largest first), so that at
+// every step of pass B the lanes of a wavefront hold particles with similar
+// bond counts and the bond loop wastes few lanes.  Pass B adds each
 // particle's kinetic term and integrates its bonds; the next bond entry and
 // the next particle's data are loaded before the current ones are used, so
 // memory latency overlaps the arithmetic.  A full list is processed early.
@@ -101,7 +105,7 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
         else if constexpr (PACK) return (int)((unsigned)lp[k][lane] >> 26);
         else return lnb[k][lane];
     };
-    const int t = blockIdx.x * BLK + lane;
+    const int t = xcd_tile(blockIdx.x, gridDim.x) * BLK + lane;
     const bool live = t < G;
     const bool KIN = skin != nullptr;
     const double *sp = spos + (size_t)f * n * PS;
@@ -181,22 +185,25 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                     if constexpr (!PACK) lnb[m + 1][lane] = vs;
                 }
             }
-            int p = getp(0);
-            double pos[PS], kk[KN];
+            // particle data and first bond entry of list entry q
+            auto fetch = [&](int q, double (&ps)[PS], double (&kv)[KN], ER (&e0)[ES]) {
+                const int pp = getp(q);
 #pragma unroll
-            for (int a = 0; a < PS; ++a) pos[a] = sp[(size_t)p * PS + a];
-            if (KIN)
-#pragma unroll
-                for (int k = 0; k < KN; ++k) kk[k] = sk[(size_t)p * KN + k];
-            for (int q = 0; q < cnt; ++q) {
-                // next particle's data, loaded ahead
-                const int pn = getp(q + 1 < cnt ? q + 1 : q);
-                double npos[PS], nkk[KN];
-#pragma unroll
-                for (int a = 0; a < PS; ++a) npos[a] = sp[(size_t)pn * PS + a];
+                for (int a = 0; a < PS; ++a) ps[a] = sp[(size_t)pp * PS + a];
                 if (KIN)
 #pragma unroll
-                    for (int k = 0; k < KN; ++k) nkk[k] = sk[(size_t)pn * KN + k];
+                    for (int k = 0; k < KN; ++k) kv[k] = sk[(size_t)pp * KN + k];
+                if constexpr (PAIRS) load_entry<ER, ES>(bd + (size_t)pp * cap * ES, e0);
+            };
+            double pos[PS], kk[KN];
+            ER E0[ES];
+            fetch(0, pos, kk, E0);
+            for (int q = 0; q < cnt; ++q) {
+                const int p = getp(q);
+                // the next particle's data, loaded ahead
+                double npos[PS], nkk[KN];
+                ER nE0[ES];
+                fetch(q + 1 < cnt ? q + 1 : q, npos, nkk, nE0);
 
                 const double a0 = minimg(pos[0] - x0, g.L[0], g.invL[0], g.periodic[0]);
                 const double a1 = minimg(pos[1] - x1, g.L[1], g.invL[1], g.periodic[1]);
@@ -210,30 +217,36 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                 if constexpr (PAIRS) {
                     const int nb = getnb(q);
                     ncontrib += nb;
-                    if (nb > 0) {
-                        const ER ta0 = (ER)(a0 * s2), ta1 = (ER)(a1 * s2), ta2 = (ER)(a2 * s2), Ae = (ER)A;
-                        const ER *ptr = bd + (size_t)p * cap * ES;
-                        ER E[ES];
-                        load_entry<ER, ES>(ptr, E);
-                        for (int j = 0; j < nb; ++j) {
-                            ER En[ES];
-                            load_entry<ER, ES>(ptr + (size_t)(j + 1 < nb ? j + 1 : j) * ES, En);
-                            integrate(E, ta0, ta1, ta2, Ae);
-#pragma unroll
-                            for (int k = 0; k < ES; ++k) E[k] = En[k];
-                        }
+                    const ER ta0 = (ER)(a0 * s2), ta1 = (ER)(a1 * s2), ta2 = (ER)(a2 * s2), Ae = (ER)A;
+                    const ER *ptr = bd + (size_t)p * cap * ES;
+                    // entries j and j+1 alternate between E0 and E1, each
+                    // loaded one step before it is used
+                    ER E1[ES];
+                    int j = 0;
+                    for (; j + 1 < nb; j += 2) {
+                        load_entry<ER, ES>(ptr + (size_t)(j + 1) * ES, E1);
+                        integrate(E0, ta0, ta1, ta2, Ae);
+                        load_entry<ER, ES>(ptr + (size_t)(j + 2 < nb ? j + 2 : j + 1) * ES, E0);
+                        integrate(E1, ta0, ta1, ta2, Ae);
                     }
+                    if (j < nb) integrate(E0, ta0, ta1, ta2, Ae);
                 }
-                p = pn;
 #pragma unroll
                 for (int a = 0; a < PS; ++a) pos[a] = npos[a];
                 if (KIN)
 #pragma unroll
                     for (int k = 0; k < KN; ++k) kk[k] = nkk[k];
+                if constexpr (PAIRS)
+#pragma unroll
+                    for (int k = 0; k < ES; ++k) E0[k] = nE0[k];
             }
         };
 
-        // pass A
+        // pass A: candidates in batches of NB, the batch's loads issued
+        // together (a load whose result the next step needs would hold the
+        // lane for a full memory latency per candidate)
+        constexpr int NB = 8;
+        static_assert(MAXL >= NB, "list shorter than a batch");
         const int cx = cell_coord(g, 0, x0), cy = cell_coord(g, 1, x1), cz = cell_coord(g, 2, x2);
         int rows[9];
         const int nrows = neighbour_rows(g, cy, cz, rows);
@@ -242,34 +255,50 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
             int rb[2], re[2];
             const int nr = row_ranges(g, st, rows[r], cx, rb, re);
             for (int s = 0; s < nr; ++s) {
-                for (int p = rb[s]; p < re[s]; ++p) {
-                    const double a0 = minimg(sp[(size_t)p * PS] - x0, g.L[0], g.invL[0], g.periodic[0]);
-                    const double a1 = minimg(sp[(size_t)p * PS + 1] - x1, g.L[1], g.invL[1], g.periodic[1]);
-                    const double a2 = (D == 3) ? minimg(sp[(size_t)p * PS + 2] - x2, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
-                    const double asq = a0 * a0 + a1 * a1 + a2 * a2;
-                    if (!(asq < g.rc2)) continue;
-                    ++nball;
-                    if constexpr (PAIRS) {
-                        const int nb = min(bc[p], cap);
-                        if constexpr (PACK) {
-                            lp[cnt][lane] = p | (nb << 26);
-                        } else {
-                            lp[cnt][lane] = p;
-                            lnb[cnt][lane] = (unsigned short)nb;
-                        }
-                    } else {
-                        lp[cnt][lane] = p;
-                    }
-                    ++cnt;
-                    if (cnt == MAXL) {
+                for (int p0 = rb[s]; p0 < re[s]; p0 += NB) {
+                    if (cnt > MAXL - NB) {  // room for a whole batch
                         process(cnt);
                         cnt = 0;
+                    }
+                    double q[NB][PS];
+                    int qb[NB];
+#pragma unroll
+                    for (int u = 0; u < NB; ++u) {
+                        const int p = min(p0 + u, re[s] - 1);
+#pragma unroll
+                        for (int a = 0; a < PS; ++a) q[u][a] = sp[(size_t)p * PS + a];
+                        if constexpr (PAIRS) qb[u] = bc[p];
+                    }
+#pragma unroll
+                    for (int u = 0; u < NB; ++u) {
+                        const int p = p0 + u;
+                        if (p >= re[s]) break;
+                        const double a0 = minimg(q[u][0] - x0, g.L[0], g.invL[0], g.periodic[0]);
+                        const double a1 = minimg(q[u][1] - x1, g.L[1], g.invL[1], g.periodic[1]);
+                        const double a2 = (D == 3) ? minimg(q[u][2] - x2, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                        const double asq = a0 * a0 + a1 * a1 + a2 * a2;
+                        if (!(asq < g.rc2)) continue;
+                        ++nball;
+                        if constexpr (PAIRS) {
+                            const int nb = min(qb[u], cap);
+                            if constexpr (PACK) {
+                                lp[cnt][lane] = p | (nb << 26);
+                            } else {
+                                lp[cnt][lane] = p;
+                                lnb[cnt][lane] = (unsigned short)nb;
+                            }
+                        } else {
+                            lp[cnt][lane] = p;
+                        }
+                        ++cnt;
                     }
                 }
             }
         }
         if (cnt > 0) process(cnt);
+    }
 
+    if (live) {
         const size_t ob = ((size_t)f * G + gp) * OD * OD;
         auto store = [&](double *dst, const double *acc, double scale) {
             for (int a = 0; a < OD; ++a) dst[ob + a + OD * a] = acc[a] * scale;
@@ -286,12 +315,13 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
         if (ELA) store(out.ela, ae, 1.0);
         if (KIN) store(out.kin, ak, cgnorm);
     }
-    // counts: one atomic per wavefront
+    // counts: one plain store per wavefront
     nball = wave_sum(nball);
     ncontrib = wave_sum(ncontrib);
     if ((lane & 63) == 0) {
-        atomicAdd(out.n_ball, nball);
-        atomicAdd(out.n_contrib, ncontrib);
+        const size_t w = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (BLK / 64) + (lane >> 6);
+        out.wave_counts[2 * w] = nball;
+        out.wave_counts[2 * w + 1] = ncontrib;
     }
 }
 

@@ -93,6 +93,18 @@ __device__ __forceinline__ int neighbour_rows(const Geom &g, int cy, int cz, int
     return n;
 }
 
+// Workgroups are dealt to the 8 XCDs of the chip round-robin by linear
+// workgroup id, and each XCD has its own L2.  With nx blocks per frame row
+// (nx a multiple of 8, so the XCD of block x is x mod 8), block x is given
+// the tile (x mod 8) * nx/8 + x / 8: each XCD then works through one
+// contiguous eighth of the frame's tiles (grid points in Morton order,
+// particles in cell order, so a compact piece of space) and its L2 holds
+// only that piece's particles and bonds.
+__device__ __forceinline__ int xcd_tile(int x, int nx)
+{
+    return (nx & 7) ? x : (x & 7) * (nx >> 3) + (x >> 3);
+}
+
 // Sum over the 64 lanes of a wavefront (all lanes must be active).
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
@@ -200,6 +212,46 @@ __global__ void k_cell_rank(int n, int ncell, const int *__restrict__ pcell, con
     }
     if (sdisp)
         for (int a = 0; a < 3; ++a) sdisp[fk * 3 + a] = disp ? disp[fi * 3 + a] : 0.0;
+}
+
+// Fold the per-wavefront statistics of one batch into the job totals: sums
+// of (balls, contributions) into stats[0..1], the largest bond count into
+// maxcount[0].  One workgroup; runs after the batch on the same stream.
+__global__ __launch_bounds__(1024) void k_reduce_stats(int nw, const unsigned long long *__restrict__ wave_counts,
+                                                       int nm, const int *__restrict__ wave_max,
+                                                       unsigned long long *__restrict__ stats, int *__restrict__ maxcount)
+{
+    __shared__ unsigned long long s0[16], s1[16];
+    __shared__ int sm[16];
+    unsigned long long a = 0, b = 0;
+    int m = 0;
+    for (int k = threadIdx.x; k < nw; k += blockDim.x) {
+        a += wave_counts[2 * k];
+        b += wave_counts[2 * k + 1];
+    }
+    for (int k = threadIdx.x; k < nm; k += blockDim.x) m = max(m, wave_max[k]);
+    a = wave_sum(a);
+    b = wave_sum(b);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s0[w] = a;
+        s1[w] = b;
+        sm[w] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nwv = blockDim.x >> 6;
+        for (int k = 1; k < nwv; ++k) {
+            a += s0[k];
+            b += s1[k];
+            m = max(m, sm[k]);
+        }
+        stats[0] += a;
+        stats[1] += b;
+        maxcount[0] = max(maxcount[0], m);
+    }
 }
 
 }  // namespace pgt

@@ -227,6 +227,42 @@ def test_batched_frames_match_single(gpu):
         assert np.array_equal(out_k[f].cpu().numpy(), single["kinetic"])
 
 
+def test_batched_frames_all_fields_match_single(gpu):
+    """Twenty frames in one launch (masses, velocities and displacements from
+    device memory) against the same frames one at a time through the host
+    API: every field must agree bit for bit (a fixed summation order does not
+    depend on how frames are batched)."""
+    import torch
+
+    nfr = 20
+    frames = [workload.config_frame(1, f) for f in range(nfr)]
+    grid = frames[0][1]
+    n = len(frames[0][0]["diam"])
+    L = frames[0][0]["L"]
+    rng = np.random.default_rng(11)
+    disp = [np.concatenate([rng.normal(0, 0.02, (n, 2)), np.zeros((n, 1))], axis=1) for _ in range(nfr)]
+    e = engine.Engine(dim=2, max_rcut=max(1.25 * 1.62, 1.62))
+    e.set_box(L, [1, 1, 0])
+    e.set_grid(grid)
+    dev = torch.device("cuda:0")
+    stack = lambda key: torch.tensor(np.stack([f[0][key] for f in frames]), dtype=torch.float64, device=dev)
+    pos, diam, mass, vel = stack("pos"), stack("diam"), stack("mass"), stack("vel")
+    dsp = torch.tensor(np.stack(disp), dtype=torch.float64, device=dev)
+    outs = {k: torch.zeros((nfr, len(grid), 2, 2), dtype=torch.float64, device=dev) for k in ("v", "k", "e")}
+    torch.cuda.synchronize()
+    e.set_tuning(frames_per_launch=nfr)
+    e.compute_frames(nfr, n, pos, diam, mass, vel, dsp, fields=("virial", "kinetic", "elastic"),
+                     d_virial=outs["v"], d_kinetic=outs["k"], d_elastic=outs["e"])
+    torch.cuda.synchronize()
+    for f in (0, 7, nfr - 1):
+        fr = frames[f][0]
+        e.set_particles(fr["pos"], fr["diam"], fr["mass"], fr["vel"], disp[f])
+        single = e.compute(fields=("virial", "kinetic", "elastic"))
+        assert np.array_equal(outs["v"][f].cpu().numpy(), single["virial"])
+        assert np.array_equal(outs["k"][f].cpu().numpy(), single["kinetic"])
+        assert np.array_equal(outs["e"][f].cpu().numpy(), single["elastic"])
+
+
 def test_fp32_form_error_reported(gpu):
     """FP32 forces and kernel weights with FP64 minimum image and accumulation:
     the error vs the FP64 oracle is measured, not assumed."""
