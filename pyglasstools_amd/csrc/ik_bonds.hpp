@@ -11,19 +11,34 @@ namespace pgt {
 
 // Entries of particle p (cell-sorted index) sit at bonds[(f*n + p)*cap*ES ...],
 // one per neighbour j within max_rcut with a nonzero pair force (or, with the
-// elastic field, a nonzero force at r_ij + u_ij).  Layout, in units of ER:
+// elastic field, a nonzero force at r_ij + u_ij).  N_cg is the kernel
+// normalisation and w the quadrature sum of the unnormalised kernel shape.
+//
+// FP64 (ER = double), ES = D + 1 (+ KN) values, no padding:
 //   [0, D)     b' = r_ij / rc                   (scaled bond vector)
-//   [D]        C' = |r_ij|^2 / rc^2
-//   [D+1]      c' = 0.5 * N_cg * (F/r) * rc^2   (virial: T += c' w b' (x) b')
-//   [D+2, ..)  elastic: 0.5 * N_cg * (F1_a (u+r)_b - F_a r_b), packed symmetric
-// padded to a 16-byte multiple.  N_cg is the kernel normalisation and w the
-// quadrature sum of the unnormalised kernel shape.
+//   [D]        c' = 0.5 * N_cg * (F/r) * rc^2   (virial: T += c' w b' (x) b')
+//   [D+1, ..)  elastic: 0.5 * N_cg * (F1_a (u+r)_b - F_a r_b), packed symmetric
+// The gather reads entries two at a time (a group of 2*ES doubles, a whole
+// number of 16-byte pieces, 16-byte aligned since cap is even) and
+// recomputes |b'|^2 itself: the memory path of a CU returns about one
+// scattered 16-byte piece per lane per cycle, so bytes per bond set the
+// gather's speed.
+//
+// FP32 (ER = float), one entry per read, padded to 16 bytes:
+//   [0, D) b', [D] C' = |r_ij|^2 / rc^2, [D+1] c', [D+2, ..) elastic
 template <int D, int OD, bool ELA, typename ER>
 struct BondLayout {
     static constexpr int KN = OD * (OD + 1) / 2;
-    static constexpr int RAW = D + 2 + (ELA ? KN : 0);
+    static constexpr bool F64 = std::is_same<ER, double>::value;
+    static constexpr bool HAS_C2 = !F64;           // entry stores |b'|^2
+    static constexpr int OC = HAS_C2 ? D + 1 : D;  // offset of c'
+    static constexpr int OE = OC + 1;              // offset of the elastic terms
+    static constexpr int RAW = OE + (ELA ? KN : 0);
     static constexpr int PER16 = 16 / (int)sizeof(ER);
-    static constexpr int ES = ((RAW + PER16 - 1) / PER16) * PER16;
+    static constexpr int ES = F64 ? RAW : ((RAW + PER16 - 1) / PER16) * PER16;
+    static constexpr int GRP = F64 ? 2 : 1;  // entries per read group
+    static constexpr int GV = GRP * ES;      // values per group, a multiple of PER16
+    static_assert(GV % PER16 == 0, "a read group is a whole number of 16-byte pieces");
 };
 
 // One lane per particle.  Pass 1 walks the 3x3(x3) cells around the particle
@@ -60,6 +75,27 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
     ER *out = bonds + ((size_t)f * n + p) * (size_t)cap * ES;
     const double h = 0.5 * cgnorm;
     int cnt = 0;
+    ER pend[LY::RAW];  // FP64: an even entry awaiting its partner
+#pragma unroll
+    for (int k = 0; k < LY::RAW; ++k) pend[k] = ER(0);
+    auto store_group = [](ER *dst, const ER (&gv)[LY::GV]) {
+        using V = typename std::conditional<std::is_same<ER, double>::value, double2, float4>::type;
+        V *q = reinterpret_cast<V *>(dst);
+#pragma unroll
+        for (int k = 0; k < LY::GV / LY::PER16; ++k) {
+            V t;
+            if constexpr (LY::PER16 == 2) {
+                t.x = gv[2 * k];
+                t.y = gv[2 * k + 1];
+            } else {
+                t.x = gv[4 * k];
+                t.y = gv[4 * k + 1];
+                t.z = gv[4 * k + 2];
+                t.w = gv[4 * k + 3];
+            }
+            q[k] = t;
+        }
+    };
 
     // pass 2 over the first m listed candidates, NF at a time with their
     // positions and diameters loaded together
@@ -100,16 +136,36 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
                     continue;
                 }
                 if (live && cnt < cap) {
-                    ER *e = out + (size_t)cnt * ES;
-                    for (int a = 0; a < D; ++a) e[a] = (ER)(b[a] * g.inv_rc);
-                    e[D] = (ER)(rsq * g.inv_rc2);
-                    e[D + 1] = VIR ? (ER)(h * f0 / g.inv_rc2) : (ER)0;
-                    if (ELA) {
-                        ER *t = e + D + 2;
+                    ER ev[LY::RAW];
+                    for (int a = 0; a < D; ++a) ev[a] = (ER)(b[a] * g.inv_rc);
+                    if constexpr (LY::HAS_C2) ev[D] = (ER)(rsq * g.inv_rc2);
+                    ev[LY::OC] = VIR ? (ER)(h * f0 / g.inv_rc2) : (ER)0;
+                    if constexpr (ELA) {
+                        ER *t = ev + LY::OE;
                         auto T = [&](int a, int c) { return h * (f1 * ur[a] * ur[c] - f0 * b[a] * b[c]); };
                         for (int a = 0; a < OD; ++a) t[a] = (ER)T(a, a);
                         if (OD == 2) t[2] = (ER)T(0, 1);
                         else { t[3] = (ER)T(0, 1); t[4] = (ER)T(0, 2); t[5] = (ER)T(1, 2); }
+                    }
+                    if constexpr (LY::GRP == 2) {
+                        // FP64: an even entry waits in registers for its odd
+                        // partner, then both go out as 16-byte stores
+                        if ((cnt & 1) == 0) {
+#pragma unroll
+                            for (int k = 0; k < LY::RAW; ++k) pend[k] = ev[k];
+                        } else {
+                            ER gv[LY::GV];
+#pragma unroll
+                            for (int k = 0; k < LY::RAW; ++k) {
+                                gv[k] = pend[k];
+                                gv[LY::ES + k] = ev[k];
+                            }
+                            store_group(out + (size_t)(cnt - 1) * ES, gv);
+                        }
+                    } else {
+                        ER *e = out + (size_t)cnt * ES;
+#pragma unroll
+                        for (int k = 0; k < LY::RAW; ++k) e[k] = ev[k];
                     }
                 }
                 ++cnt;
@@ -155,6 +211,12 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
         }
     }
     if (m > 0) forces(m);
+    if constexpr (LY::GRP == 2)
+        if (live && (cnt & 1) && cnt <= cap) {  // the last, unpaired entry
+            ER *e = out + (size_t)(cnt - 1) * ES;
+#pragma unroll
+            for (int k = 0; k < LY::RAW; ++k) e[k] = pend[k];
+        }
 
     if (live) bcount[(size_t)f * n + p] = cnt;
     int mc = live ? cnt : 0;

@@ -136,4 +136,151 @@ struct GLRule {
     double w[kMaxQArg];
 };
 
+// Adaptive quadrature (AdaptiveCGFunc::getBondFunc, CoarseGrainFunction.h:
+// 124-127 -> gsl_quad::integrate, Quadrature.h:39-51: gsl_integration_qag
+// with key 1, the 15-point Gauss-Kronrod rule).  The rule's abscissae and
+// weights are the published QUADPACK constants.  QAG: apply the rule on
+// [0, 1]; stop if the error estimate meets max(epsabs, epsrel |result|) (or
+// is at the rounding floor); otherwise bisect the interval with the largest
+// error estimate until the summed error meets the tolerance or `limit`
+// intervals exist, and return the sum of the interval results in storage
+// order.  Up to kQagMax intervals are kept per call; a call that would need
+// more (limit > kQagMax and not converged) sets `capped`.
+constexpr int kQagMax = 128;
+
+struct AdaptQ {
+    int limit;
+    double epsabs, epsrel;
+};
+
+__device__ __forceinline__ double gk15_x(int k)
+{
+    constexpr double v[8] = {0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
+                             0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
+                             0.586087235467691130294144845693013, 0.405845151377397166906606412076961,
+                             0.207784955007898467600689403773245, 0.000000000000000000000000000000000};
+    return v[k];
+}
+__device__ __forceinline__ double gk15_wk(int k)
+{
+    constexpr double v[8] = {0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
+                             0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
+                             0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
+                             0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
+    return v[k];
+}
+__device__ __forceinline__ double gk15_wg(int k)
+{
+    constexpr double v[4] = {0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
+                             0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
+    return v[k];
+}
+
+// The 15-point Gauss-Kronrod rule on [a, b] with QUADPACK's error estimate.
+template <typename F>
+__device__ void gk15_dev(const F &f, double a, double b, double &result, double &abserr, double &resabs,
+                         double &resasc)
+{
+    const double center = 0.5 * (a + b), half = 0.5 * (b - a), abs_half = fabs(half);
+    const double fc = f(center);
+    double resg = fc * gk15_wg(3);
+    double resk = fc * gk15_wk(7);
+    double ra = fabs(resk);
+    double fv1[7], fv2[7];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int jtw = 2 * j + 1;
+        const double absc = half * gk15_x(jtw);
+        const double f1 = f(center - absc), f2 = f(center + absc);
+        fv1[jtw] = f1;
+        fv2[jtw] = f2;
+        resg += gk15_wg(j) * (f1 + f2);
+        resk += gk15_wk(jtw) * (f1 + f2);
+        ra += gk15_wk(jtw) * (fabs(f1) + fabs(f2));
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int jtwm1 = 2 * j;
+        const double absc = half * gk15_x(jtwm1);
+        const double f1 = f(center - absc), f2 = f(center + absc);
+        fv1[jtwm1] = f1;
+        fv2[jtwm1] = f2;
+        resk += gk15_wk(jtwm1) * (f1 + f2);
+        ra += gk15_wk(jtwm1) * (fabs(f1) + fabs(f2));
+    }
+    const double mean = resk * 0.5;
+    double rasc = gk15_wk(7) * fabs(fc - mean);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) rasc += gk15_wk(j) * (fabs(fv1[j] - mean) + fabs(fv2[j] - mean));
+    double err = fabs((resk - resg) * half);
+    rasc *= abs_half;
+    ra *= abs_half;
+    if (rasc != 0.0 && err != 0.0) {
+        const double scale = pow(200.0 * err / rasc, 1.5);
+        err = (scale < 1.0) ? rasc * scale : rasc;
+    }
+    if (ra > 2.2250738585072014e-308 / (50 * 2.2204460492503131e-16)) {
+        const double min_err = 50 * 2.2204460492503131e-16 * ra;
+        if (min_err > err) err = min_err;
+    }
+    result = resk * half;
+    abserr = err;
+    resabs = ra;
+    resasc = rasc;
+}
+
+template <typename F>
+__device__ double qag_dev(const F &f, const AdaptQ &Q, bool &capped)
+{
+    double al[kQagMax], bl[kQagMax], rl[kQagMax], el[kQagMax];
+    const int limit = Q.limit < 1 ? 1 : Q.limit;
+    const int lim = limit < kQagMax ? limit : kQagMax;
+    double result, abserr, resabs, resasc;
+    gk15_dev(f, 0.0, 1.0, result, abserr, resabs, resasc);
+    al[0] = 0.0;
+    bl[0] = 1.0;
+    rl[0] = result;
+    el[0] = abserr;
+    int n = 1;
+    double tol = fmax(Q.epsabs, Q.epsrel * fabs(result));
+    const double round_off = 50 * 2.2204460492503131e-16 * resabs;
+    if ((abserr <= round_off && abserr > tol) || abserr <= tol || limit == 1) return result;
+    double area = result, errsum = abserr;
+    while (n < lim) {
+        int imax = 0;
+        for (int k = 1; k < n; ++k)
+            if (el[k] > el[imax]) imax = k;
+        const double a1 = al[imax], b2 = bl[imax];
+        const double a2 = 0.5 * (a1 + b2), b1 = a2;
+        double r1, e1, ra1, rs1, r2, e2, ra2, rs2;
+        gk15_dev(f, a1, b1, r1, e1, ra1, rs1);
+        gk15_dev(f, a2, b2, r2, e2, ra2, rs2);
+        area += r1 + r2 - rl[imax];
+        errsum += e1 + e2 - el[imax];
+        al[imax] = a1;
+        bl[imax] = b1;
+        rl[imax] = r1;
+        el[imax] = e1;
+        al[n] = a2;
+        bl[n] = b2;
+        rl[n] = r2;
+        el[n] = e2;
+        ++n;
+        tol = fmax(Q.epsabs, Q.epsrel * fabs(area));
+        if (errsum <= tol) break;
+    }
+    if (n == lim && lim < limit && errsum > tol) capped = true;
+    double sum = 0.0;
+    for (int k = 0; k < n; ++k) sum += rl[k];
+    return sum;
+}
+
+// The kernel shape (t = |dr|^2 / rc^2, no normalisation) chosen at run time.
+__device__ __forceinline__ double shape_rt(int cg, double t)
+{
+    if (cg == 0) return shape<0, double>(t);
+    if (cg == 1) return shape<1, double>(t);
+    return shape<2, double>(t);
+}
+
 }  // namespace pgt

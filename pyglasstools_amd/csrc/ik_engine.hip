@@ -70,7 +70,8 @@ __global__ void k_eval_force(PotParams P, int n, const double *rij, const double
 // is Delta(|x - (ri + s rij)|^2) (CoarseGrainFunction.h:42-45), summed over
 // the unfolded Gauss-Legendre rule.
 __global__ void k_eval_cg(int cg, double rc, int mode, int n, const double *s, const double *x, const double *ri,
-                          const double *rij, const double *gld, int nq, double *out)
+                          const double *rij, const double *gld, int nq, int adapt, AdaptQ Q, int *capped,
+                          double *out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -85,7 +86,11 @@ __global__ void k_eval_cg(int cg, double rc, int mode, int n, const double *s, c
     };
     if (mode == 0) out[i] = delta_at(0.0);
     else if (mode == 1) out[i] = delta_at(s[i]);
-    else {
+    else if (adapt) {
+        bool cp = false;
+        out[i] = qag_dev(delta_at, Q, cp);
+        if (cp) *capped = 1;
+    } else {
         double acc = 0.0;
         for (int k = 0; k < nq; ++k) acc += gld[nq + k] * delta_at(gld[k]);
         out[i] = acc;
@@ -278,7 +283,11 @@ int validate_params(const pgt_ik_params *p)
     if (p->quad_mode == PGT_QUAD_FIXED) {
         if (p->quad_order < 1 || p->quad_order > 1024) return fail(PGT_ERR_ARG, "quadrature order must be in 1..1024");
     } else if (p->quad_mode == PGT_QUAD_ADAPTIVE) {
-        return fail(PGT_ERR_UNSUPPORTED, "adaptive quadrature is not available in the field engine");
+        if (p->adapt_limit < 1) return fail(PGT_ERR_ARG, "adaptive limit (maxiter) must be at least 1");
+        if (!(p->adapt_epsabs >= 0) || !(p->adapt_epsrel >= 0))
+            return fail(PGT_ERR_ARG, "adaptive tolerances must be non-negative");
+        if (p->precision == PGT_PREC_FP32)
+            return fail(PGT_ERR_UNSUPPORTED, "the adaptive rule runs in FP64 only");
     } else {
         return fail(PGT_ERR_ARG, "unknown quadrature mode");
     }
@@ -469,7 +478,8 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     c.n = n;
     c.G = G;
     c.nf = nf;
-    c.block = e->block;
+    const bool adaptive = e->prm.quad_mode == PGT_QUAD_ADAPTIVE && pairs;
+    c.block = adaptive ? 128 : e->block;  // k_gather_adaptive runs 128-thread workgroups
     c.grid = e->d_grid.p;
     c.perm = e->d_perm.p;
     c.R.n = (int)e->gl_s.size();
@@ -518,7 +528,26 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     c.pack = n < (1 << 26) && c.cap <= 63;
     if (ev_gather) HIPCHK(hipEventRecord(ev_gather, s));
     const int cg = e->prm.cg_kind;
-    if (D == 2) {
+    if (adaptive) {
+        const AdaptQ Q{e->prm.adapt_limit, e->prm.adapt_epsabs, e->prm.adapt_epsrel};
+        const dim3 ag((unsigned)((c.G + 127) / 128), (unsigned)c.nf);
+        const bool V = fields & PGT_FIELD_VIRIAL, E = fields & PGT_FIELD_ELASTIC;
+#define PGT_LAUNCH_ADAPT(DD, OO, VV, EE)                                                                          \
+    hipLaunchKernelGGL((k_gather_adaptive<DD, OO, VV, EE>), ag, dim3(128), 0, s, c.g, c.n, c.G, c.grid, c.perm, cg, \
+                       Q, c.cgnorm, c.cstart, c.spos, c.skin, c.cap, c.bcount, (const double *)c.bonds, c.out,       \
+                       e->d_maxcount.p + 1)
+#define PGT_LAUNCH_ADAPT_F(DD, OO)                              \
+    do {                                                        \
+        if (V && E) PGT_LAUNCH_ADAPT(DD, OO, true, true);       \
+        else if (E) PGT_LAUNCH_ADAPT(DD, OO, false, true);      \
+        else PGT_LAUNCH_ADAPT(DD, OO, true, false);             \
+    } while (0)
+        if (D == 2) PGT_LAUNCH_ADAPT_F(2, 2);
+        else if (OD == 2) PGT_LAUNCH_ADAPT_F(3, 2);
+        else PGT_LAUNCH_ADAPT_F(3, 3);
+#undef PGT_LAUNCH_ADAPT_F
+#undef PGT_LAUNCH_ADAPT
+    } else if (D == 2) {
         if (fp32) launch_gather_c<2, 2, float>(c, fields, cg);
         else launch_gather_c<2, 2, double>(c, fields, cg);
     } else if (OD == 2) {
@@ -529,7 +558,8 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     }
     HIPCHK(hipGetLastError());
     if (ev_end) HIPCHK(hipEventRecord(ev_end, s));
-    hipLaunchKernelGGL(k_reduce_stats, dim3(1), dim3(1024), 0, s, nw_gather, e->d_wcounts.p, nw_bonds, e->d_wmax.p,
+    const int nrb = std::max(1, std::min(256, (std::max(nw_gather, nw_bonds) + 4095) / 4096));
+    hipLaunchKernelGGL(k_reduce_stats, dim3(nrb), dim3(1024), 0, s, nw_gather, e->d_wcounts.p, nw_bonds, e->d_wmax.p,
                        e->d_stats.p, e->d_maxcount.p);
     HIPCHK(hipGetLastError());
     return 0;
@@ -553,10 +583,11 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
     const int nb = (fs.nframes + fpl - 1) / fpl;
     int rc;
     if ((rc = ensure_events(e, (size_t)nb * 3))) return rc;
-    if ((rc = e->d_maxcount.ensure(1)) || (rc = e->d_stats.ensure(2))) return rc;
+    if ((rc = e->d_maxcount.ensure(2)) || (rc = e->d_stats.ensure(2))) return rc;
     int mx = 0;
     for (int attempt = 0; attempt < 3; ++attempt) {
-        HIPCHK(hipMemsetAsync(e->d_maxcount.p, 0, sizeof(int), e->stream));
+        // [0] largest bond count, [1] an adaptive integral ran out of workspace
+        HIPCHK(hipMemsetAsync(e->d_maxcount.p, 0, 2 * sizeof(int), e->stream));
         HIPCHK(hipMemsetAsync(e->d_stats.p, 0, sizeof(unsigned long long) * 2, e->stream));
         for (int b = 0; b < nb; ++b) {
             const int f0 = b * fpl;
@@ -569,9 +600,13 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
                             e->ev[3 * b + 1], e->ev[3 * b + 2]);
             if (rc) return rc;
         }
-        mx = 0;
-        HIPCHK(hipMemcpyAsync(&mx, e->d_maxcount.p, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+        int flags[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(flags, e->d_maxcount.p, sizeof(flags), hipMemcpyDeviceToHost, e->stream));
         HIPCHK(hipStreamSynchronize(e->stream));
+        mx = flags[0];
+        if (flags[1])
+            return fail(PGT_ERR_RANGE, "an adaptive bond integral needed more than " + std::to_string(kQagMax) +
+                                           " intervals (the GPU rule's workspace)");
         if (mx <= e->cap) break;
         if (mx > 65535) return fail(PGT_ERR_RANGE, "a particle has more than 65535 interacting neighbours");
         e->cap = ((mx + 7) / 8) * 8;
@@ -631,7 +666,7 @@ int pgt_ik_create(const pgt_ik_params *params, pgt_ik_engine **out)
         return fail(PGT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
     e->stream = e->own_stream;
-    if (gl_rule(params->quad_order, e->gl_s, e->gl_w) != 0) {
+    if (gl_rule(params->quad_mode == PGT_QUAD_FIXED ? params->quad_order : 5, e->gl_s, e->gl_w) != 0) {
         pgt_ik_destroy(e);
         return fail(PGT_ERR_ARG, "bad quadrature order");
     }
@@ -911,26 +946,29 @@ int pgt_eval_cgfunc(int cg_kind, double cg_rcut, int quad_mode, int quad_order, 
     if (!x || !ri || !out || (mode >= 1 && !rij) || (mode == 1 && !s)) return fail(PGT_ERR_ARG, "null input");
     std::vector<double> packed;
     int nq = 0;
-    if (mode == 2) {
-        if (quad_mode != PGT_QUAD_FIXED)
-            return fail(PGT_ERR_UNSUPPORTED, "adaptive quadrature is not available in the evaluator");
+    const int adapt = mode == 2 && quad_mode == PGT_QUAD_ADAPTIVE;
+    AdaptQ Q{adapt_limit, adapt_epsabs, adapt_epsrel};
+    if (mode == 2 && quad_mode != PGT_QUAD_FIXED && quad_mode != PGT_QUAD_ADAPTIVE)
+        return fail(PGT_ERR_ARG, "unknown quadrature mode");
+    if (adapt && (adapt_limit < 1 || !(adapt_epsabs >= 0) || !(adapt_epsrel >= 0)))
+        return fail(PGT_ERR_ARG, "adaptive limit must be >= 1 and tolerances non-negative");
+    if (mode == 2 && !adapt) {
         std::vector<double> gs, gw;
         if (gl_rule(quad_order, gs, gw) != 0) return fail(PGT_ERR_ARG, "quadrature order must be in 1..1024");
         nq = quad_order;
         packed = gs;
         packed.insert(packed.end(), gw.begin(), gw.end());
     }
-    (void)adapt_limit;
-    (void)adapt_epsrel;
-    (void)adapt_epsabs;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(PGT_ERR_HIP, "no HIP device available: kernel functions are evaluated on the GPU");
     double *d = nullptr;
-    const size_t tot = (size_t)n * 11 + packed.size();
+    const size_t tot = (size_t)n * 11 + packed.size() + 1;  // + the capped flag
     HIPCHK(hipMalloc(&d, sizeof(double) * tot));
     double *dx = d, *dri = d + 3 * n, *drij = d + 6 * n, *ds = d + 9 * n, *dout = d + 10 * n, *dgl = d + 11 * n;
-    hipError_t he = hipMemcpy(dx, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice);
+    int *dcap = reinterpret_cast<int *>(d + tot - 1);
+    hipError_t he = hipMemset(dcap, 0, sizeof(int));
+    if (he == hipSuccess) he = hipMemcpy(dx, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice);
     if (he == hipSuccess) he = hipMemcpy(dri, ri, sizeof(double) * 3 * n, hipMemcpyHostToDevice);
     if (he == hipSuccess && rij) he = hipMemcpy(drij, rij, sizeof(double) * 3 * n, hipMemcpyHostToDevice);
     if (he == hipSuccess && s) he = hipMemcpy(ds, s, sizeof(double) * n, hipMemcpyHostToDevice);
@@ -938,13 +976,18 @@ int pgt_eval_cgfunc(int cg_kind, double cg_rcut, int quad_mode, int quad_order, 
         he = hipMemcpy(dgl, packed.data(), sizeof(double) * packed.size(), hipMemcpyHostToDevice);
     if (he == hipSuccess) {
         hipLaunchKernelGGL(k_eval_cg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, cg_kind, cg_rcut, mode,
-                           (int)n, s ? ds : nullptr, dx, dri, rij ? drij : nullptr, dgl, nq, dout);
+                           (int)n, s ? ds : nullptr, dx, dri, rij ? drij : nullptr, dgl, nq, adapt, Q, dcap, dout);
         he = hipGetLastError();
     }
     if (he == hipSuccess) he = hipDeviceSynchronize();
     if (he == hipSuccess) he = hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost);
+    int capped = 0;
+    if (he == hipSuccess) he = hipMemcpy(&capped, dcap, sizeof(int), hipMemcpyDeviceToHost);
     (void)hipFree(d);
     if (he != hipSuccess) return fail(PGT_ERR_HIP, std::string("kernel evaluation: ") + hipGetErrorString(he));
+    if (capped)
+        return fail(PGT_ERR_RANGE, "an adaptive bond integral needed more than " + std::to_string(kQagMax) +
+                                       " intervals (the GPU rule's workspace)");
     return PGT_OK;
 }
 

@@ -43,7 +43,7 @@ __device__ __forceinline__ ER bond_quad(const GLRule &R, const double *__restric
     return acc;
 }
 
-// One bond entry (ES values, a 16-byte multiple) in 16-byte loads.
+// N values (a whole number of 16-byte pieces, 16-byte aligned) in 16-byte loads.
 template <typename ER, int ES>
 __device__ __forceinline__ void load_entry(const ER *__restrict__ p, ER (&v)[ES])
 {
@@ -92,7 +92,10 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
     constexpr int PS = (D == 2) ? 2 : 4;
     constexpr int KN = OD * (OD + 1) / 2;
     constexpr bool PAIRS = VIR || ELA;
-    constexpr int ES = BondLayout<D, OD, ELA, ER>::ES;
+    using LY = BondLayout<D, OD, ELA, ER>;
+    constexpr int ES = LY::ES;    // values per entry
+    constexpr int GRP = LY::GRP;  // entries per read group
+    constexpr int GV = LY::GV;    // values per read group
     __shared__ int lp[MAXL][BLK];
     __shared__ unsigned short lnb[(PAIRS && !PACK) ? MAXL : 1][BLK];
 
@@ -125,12 +128,24 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
         const double x2 = (D == 3) ? grid[(size_t)gp * 3 + 2] : 0.0;
         const double s2 = 2.0 * g.inv_rc;
 
-        auto integrate = [&](const ER (&e)[ES], ER ta0, ER ta1, ER ta2, ER A) {
+        // integrate the entry at offset O (a std::integral_constant) of a
+        // read group
+        auto integrate = [&](const ER (&G)[GV], auto O, ER ta0, ER ta1, ER ta2, ER A) {
+            constexpr int o = decltype(O)::value;
+            const ER *e = G + o;
             ER B = ta0 * e[0] + ta1 * e[1];
             if (D == 3) B = fma(ta2, e[2], B);
-            const ER w = bond_quad<CG, NQ, ER>(R, gld, R.n, A, B, e[D]);
+            ER C;
+            if constexpr (LY::HAS_C2) {
+                C = e[D];
+            } else {
+                C = e[0] * e[0];
+                C = fma(e[1], e[1], C);
+                if (D == 3) C = fma(e[2], e[2], C);
+            }
+            const ER w = bond_quad<CG, NQ, ER>(R, gld, R.n, A, B, C);
             if (VIR) {
-                const ER c = w * e[D + 1];
+                const ER c = w * e[LY::OC];
                 const ER cb0 = c * e[0], cb1 = c * e[1];
                 if constexpr (std::is_same<ER, double>::value) {
                     av[0] = fma(cb0, e[0], av[0]);
@@ -161,10 +176,16 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
             if (ELA) {
 #pragma unroll
                 for (int m = 0; m < KN; ++m) {
-                    if constexpr (std::is_same<ER, double>::value) ae[m] = fma(w, e[D + 2 + m], ae[m]);
-                    else ae[m] += (double)(w * e[D + 2 + m]);
+                    if constexpr (std::is_same<ER, double>::value) ae[m] = fma(w, e[LY::OE + m], ae[m]);
+                    else ae[m] += (double)(w * e[LY::OE + m]);
                 }
             }
+        };
+        // the entries of one group that exist (k-th group of a particle with nb)
+        auto integrate_group = [&](const ER (&G)[GV], int k, int nb, ER ta0, ER ta1, ER ta2, ER A) {
+            integrate(G, std::integral_constant<int, 0>(), ta0, ta1, ta2, A);
+            if constexpr (GRP == 2)
+                if (2 * k + 1 < nb) integrate(G, std::integral_constant<int, ES>(), ta0, ta1, ta2, A);
         };
 
         // pass B over the first cnt list entries
@@ -186,23 +207,23 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                 }
             }
             // particle data and first bond entry of list entry q
-            auto fetch = [&](int q, double (&ps)[PS], double (&kv)[KN], ER (&e0)[ES]) {
+            auto fetch = [&](int q, double (&ps)[PS], double (&kv)[KN], ER (&e0)[GV]) {
                 const int pp = getp(q);
 #pragma unroll
                 for (int a = 0; a < PS; ++a) ps[a] = sp[(size_t)pp * PS + a];
                 if (KIN)
 #pragma unroll
                     for (int k = 0; k < KN; ++k) kv[k] = sk[(size_t)pp * KN + k];
-                if constexpr (PAIRS) load_entry<ER, ES>(bd + (size_t)pp * cap * ES, e0);
+                if constexpr (PAIRS) load_entry<ER, GV>(bd + (size_t)pp * cap * ES, e0);
             };
             double pos[PS], kk[KN];
-            ER E0[ES];
+            ER E0[GV];
             fetch(0, pos, kk, E0);
             for (int q = 0; q < cnt; ++q) {
                 const int p = getp(q);
                 // the next particle's data, loaded ahead
                 double npos[PS], nkk[KN];
-                ER nE0[ES];
+                ER nE0[GV];
                 fetch(q + 1 < cnt ? q + 1 : q, npos, nkk, nE0);
 
                 const double a0 = minimg(pos[0] - x0, g.L[0], g.invL[0], g.periodic[0]);
@@ -219,17 +240,18 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                     ncontrib += nb;
                     const ER ta0 = (ER)(a0 * s2), ta1 = (ER)(a1 * s2), ta2 = (ER)(a2 * s2), Ae = (ER)A;
                     const ER *ptr = bd + (size_t)p * cap * ES;
-                    // entries j and j+1 alternate between E0 and E1, each
+                    // read groups k and k+1 alternate between E0 and E1, each
                     // loaded one step before it is used
-                    ER E1[ES];
-                    int j = 0;
-                    for (; j + 1 < nb; j += 2) {
-                        load_entry<ER, ES>(ptr + (size_t)(j + 1) * ES, E1);
-                        integrate(E0, ta0, ta1, ta2, Ae);
-                        load_entry<ER, ES>(ptr + (size_t)(j + 2 < nb ? j + 2 : j + 1) * ES, E0);
-                        integrate(E1, ta0, ta1, ta2, Ae);
+                    const int ng = (nb + GRP - 1) / GRP;
+                    ER E1[GV];
+                    int k = 0;
+                    for (; k + 1 < ng; k += 2) {
+                        load_entry<ER, GV>(ptr + (size_t)(k + 1) * GV, E1);
+                        integrate_group(E0, k, nb, ta0, ta1, ta2, Ae);
+                        load_entry<ER, GV>(ptr + (size_t)(k + 2 < ng ? k + 2 : k + 1) * GV, E0);
+                        integrate_group(E1, k + 1, nb, ta0, ta1, ta2, Ae);
                     }
-                    if (j < nb) integrate(E0, ta0, ta1, ta2, Ae);
+                    if (k < ng) integrate_group(E0, k, nb, ta0, ta1, ta2, Ae);
                 }
 #pragma unroll
                 for (int a = 0; a < PS; ++a) pos[a] = npos[a];
@@ -238,7 +260,7 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                     for (int k = 0; k < KN; ++k) kk[k] = nkk[k];
                 if constexpr (PAIRS)
 #pragma unroll
-                    for (int k = 0; k < ES; ++k) E0[k] = nE0[k];
+                    for (int k = 0; k < GV; ++k) E0[k] = nE0[k];
             }
         };
 

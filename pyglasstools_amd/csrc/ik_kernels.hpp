@@ -3,7 +3,8 @@
 //
 //   cell list   k_cell_count -> k_cell_scan -> k_cell_fill -> k_cell_rank
 //   bond list   k_bonds   (ik_bonds.hpp)
-//   fields      k_gather  (ik_gather.hpp)
+//   fields      k_gather  (ik_gather.hpp); with the adaptive bond integral
+//               k_gather_adaptive (ik_gather_adaptive.hpp)
 //
 // All per-frame arrays are stacked along blockIdx.y (one frame per y index).
 #pragma once
@@ -216,7 +217,9 @@ __global__ void k_cell_rank(int n, int ncell, const int *__restrict__ pcell, con
 
 // Fold the per-wavefront statistics of one batch into the job totals: sums
 // of (balls, contributions) into stats[0..1], the largest bond count into
-// maxcount[0].  One workgroup; runs after the batch on the same stream.
+// maxcount[0].  Each workgroup reduces a stretch of the per-wavefront arrays
+// and adds its result with one atomic per total (a few hundred atomics per
+// batch, against one per wavefront of the batch's kernels before).
 __global__ __launch_bounds__(1024) void k_reduce_stats(int nw, const unsigned long long *__restrict__ wave_counts,
                                                        int nm, const int *__restrict__ wave_max,
                                                        unsigned long long *__restrict__ stats, int *__restrict__ maxcount)
@@ -225,11 +228,12 @@ __global__ __launch_bounds__(1024) void k_reduce_stats(int nw, const unsigned lo
     __shared__ int sm[16];
     unsigned long long a = 0, b = 0;
     int m = 0;
-    for (int k = threadIdx.x; k < nw; k += blockDim.x) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nw; k += stride) {
         a += wave_counts[2 * k];
         b += wave_counts[2 * k + 1];
     }
-    for (int k = threadIdx.x; k < nm; k += blockDim.x) m = max(m, wave_max[k]);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += stride) m = max(m, wave_max[k]);
     a = wave_sum(a);
     b = wave_sum(b);
 #pragma unroll
@@ -248,9 +252,9 @@ __global__ __launch_bounds__(1024) void k_reduce_stats(int nw, const unsigned lo
             b += s1[k];
             m = max(m, sm[k]);
         }
-        stats[0] += a;
-        stats[1] += b;
-        maxcount[0] = max(maxcount[0], m);
+        if (a) atomicAdd(&stats[0], a);
+        if (b) atomicAdd(&stats[1], b);
+        if (m) atomicMax(maxcount, m);
     }
 }
 
@@ -258,3 +262,4 @@ __global__ __launch_bounds__(1024) void k_reduce_stats(int nw, const unsigned lo
 
 #include "ik_bonds.hpp"
 #include "ik_gather.hpp"
+#include "ik_gather_adaptive.hpp"

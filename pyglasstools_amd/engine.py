@@ -134,7 +134,9 @@ class Engine(object):
     """One pgt_ik_engine."""
 
     def __init__(self, dim=2, pot="poly12", params=None, scaled_rcut=1.25, cg="octic", cg_rcut=2.0, order=5,
-                 max_rcut=None, precision="fp64", diameters=None):
+                 max_rcut=None, precision="fp64", diameters=None, adaptive=None):
+        """adaptive: None for the fixed-order Gauss-Legendre rule of `order`
+        nodes, else (limit, epsrel, epsabs) of the QAG rule (AdaptiveCGFunc)."""
         if params is None:
             params = poly12_params(rcut=scaled_rcut)
         if max_rcut is None:
@@ -151,8 +153,13 @@ class Engine(object):
             p.pot_params[k] = v
         p.cg_kind = CG[cg]
         p.cg_rcut = cg_rcut
-        p.quad_mode = 0
-        p.quad_order = order
+        if adaptive is None:
+            p.quad_mode = 0
+            p.quad_order = order
+        else:
+            p.quad_mode = 1
+            p.quad_order = 0
+            p.adapt_limit, p.adapt_epsrel, p.adapt_epsabs = int(adaptive[0]), float(adaptive[1]), float(adaptive[2])
         p.precision = PREC[precision]
         p.max_rcut = max_rcut
         self.params = p
@@ -244,7 +251,9 @@ def eval_pair_force(rij, di, dj, pot="poly12", params=None, scaled_rcut=1.25):
     return f, e, r
 
 
-def eval_cgfunc(mode, x, ri, rij=None, s=None, cg="octic", rcut=2.0, order=5):
+def eval_cgfunc(mode, x, ri, rij=None, s=None, cg="octic", rcut=2.0, order=5, adaptive=None):
+    """Kernel (delta), integrand (obj) or bond integral (bond) on the GPU;
+    adaptive = (limit, epsrel, epsabs) selects the QAG rule for the bond."""
     x = _f64(x, (-1, 3))
     n = x.shape[0]
     ri = _f64(ri, (-1, 3))
@@ -252,6 +261,8 @@ def eval_cgfunc(mode, x, ri, rij=None, s=None, cg="octic", rcut=2.0, order=5):
     s = None if s is None else _f64(np.broadcast_to(s, (n,)))
     out = np.zeros(n)
     modes = {"delta": 0, "obj": 1, "bond": 2}
-    _check(lib().pgt_eval_cgfunc(CG[cg], rcut, 0, order, 0, 0.0, 0.0, modes[mode], n, _ptr(s), _ptr(x), _ptr(ri),
+    qm, lim, er, ea = (0, 0, 0.0, 0.0) if adaptive is None else (1, int(adaptive[0]), float(adaptive[1]),
+                                                                 float(adaptive[2]))
+    _check(lib().pgt_eval_cgfunc(CG[cg], rcut, qm, order, lim, er, ea, modes[mode], n, _ptr(s), _ptr(x), _ptr(ri),
                                  _ptr(rij), _ptr(out)), "pgt_eval_cgfunc")
     return out

@@ -304,3 +304,60 @@ def test_eval_cgfunc_matches_oracle(gpu):
     for k in range(0, n, 5):
         assert d[k] == pytest.approx(oracle.cg_delta("octic", 2.0, float(((x[k] - ri[k]) ** 2).sum())), rel=1e-13, abs=1e-15)
         assert b[k] == pytest.approx(oracle.bond_fixed("octic", 2.0, 5, x[k], ri[k], rij[k]), rel=1e-13, abs=1e-15)
+
+
+ADAPT = (100, 1e-10, 1e-12)  # (limit, epsrel, epsabs), as cgfunc.<kernel>(mode="adaptive", ...)
+
+
+@pytest.mark.parametrize("cg", ["octic", "rect", "mollifier"])
+def test_eval_adaptive_bond_matches_oracle(gpu, cg):
+    """QAG on the GPU follows the oracle's restatement of gsl_integration_qag
+    (GK15, bisect the worst interval): the two agree to rounding unless a
+    near-tie in the error estimates sends them down different bisections, in
+    which case both are still within the requested tolerance of the integral."""
+    rng = np.random.default_rng(21)
+    n = 1500
+    x = rng.uniform(-1, 1, (n, 3))
+    ri = x + rng.normal(0, 1.1, (n, 3))
+    rij = rng.normal(0, 1.0, (n, 3))
+    got = engine.eval_cgfunc("bond", x, ri, rij, cg=cg, adaptive=ADAPT)
+    limit, epsrel, epsabs = ADAPT
+    tight = 0
+    for k in range(0, n, 3):
+        ref = oracle.qag(cg, 2.0, x[k], ri[k], rij[k], epsabs, epsrel, limit)
+        err = abs(got[k] - ref)
+        assert err <= 2 * max(epsabs, epsrel * abs(ref)) + 1e-15, (k, got[k], ref)
+        tight += err <= 1e-13 * max(abs(ref), 1e-300) + 1e-16
+    assert tight >= 0.98 * len(range(0, n, 3))
+
+
+def test_eval_adaptive_workspace_limit_reported(gpu):
+    """A zero tolerance never converges on a segment that leaves the ball (the
+    Rect kernel jumps there): with a limit above the GPU rule's 128 intervals
+    the evaluator reports a range error instead of a truncated result; with a
+    limit within them it returns what the rule reached."""
+    x = np.zeros((4, 3))
+    ri = np.array([[1.5, 0.3, 0.0]] * 4)
+    rij = np.array([[1.1, 0.2, 0.0]] * 4)
+    with pytest.raises(engine.PGTError):
+        engine.eval_cgfunc("bond", x, ri, rij, cg="rect", adaptive=(1000, 0.0, 0.0))
+    got = engine.eval_cgfunc("bond", x, ri, rij, cg="rect", adaptive=(40, 0.0, 0.0))
+    ref = oracle.qag("rect", 2.0, x[0], ri[0], rij[0], 0.0, 0.0, 40)
+    assert got[0] == pytest.approx(ref, rel=1e-12)
+
+
+@pytest.mark.parametrize("cg", ["octic", "rect"])
+def test_adaptive_fields_match_oracle(gpu, cg):
+    sysd, grid = workload.config_frame(1, 2)
+    d = float(np.max(sysd["diam"]))
+    max_rcut = max(1.25 * d, d)
+    params = oracle.poly12_params()
+    e = engine.Engine(dim=2, params=params, cg=cg, max_rcut=max_rcut, adaptive=ADAPT)
+    e.set_system(sysd, grid)
+    got = e.compute(fields=("virial", "kinetic"))
+    st = e.stats()
+    ref = oracle.ik_compute(sysd, grid, params=params, max_rcut=max_rcut, cg=cg, adaptive=ADAPT,
+                            fields=("virial", "kinetic"), with_abs=True)
+    # each bond integral is good to max(epsabs, epsrel |I|) whatever path the
+    # bisection takes, so the field is good to that relative to its scale
+    assert_parity(got, ref, st, ("virial", "kinetic"), tol=1e-8)
