@@ -91,8 +91,104 @@ def parity_sample(systems, grid, max_rcut, got_virial, got_kinetic, stride=16):
             "kinetic_max_err_over_max": ek}
 
 
+FLOP_U_3D = 142
+PEAK_FP32_TFLOPS = 157.3
+
+
+def precision_form(args):
+    """The FP32 precision form on the 3D workload: polydisperse12, N = 131,072
+    at rho = 1 (seeds 3000..3019), 20 frames resident in HBM, virial stress on
+    a 64^3 grid, Octic r_cg = 2.0, 5-node rule.  Pair forces and kernel
+    weights in FP32; minimum-image displacements and the per-grid-point sums
+    in FP64.  Reports frames/s, contributions/s, the gather's share of the
+    FP32 VALU peak, and the per-component error against the FP64 oracle on a
+    sample of grid points of frame 0 (with the FP64 GPU form beside it)."""
+    import torch
+
+    import oracle
+    from pyglasstools_amd import engine, workload
+
+    dev = torch.device("cuda:0")
+    F, N, NG = 20, 131072, 64
+    log("generating %d 3D frames of N=%d" % (F, N))
+    systems = [workload.make_system(N, 3, 3000 + f) for f in range(F)]
+    L = systems[0]["L"]
+    grid = workload.cartesian_grid_3d(L[0], NG)
+    G = len(grid)
+    d0 = float(systems[0]["diam"].max())
+    max_rcut = max(1.25 * d0, d0)
+    pos = torch.tensor(np.stack([s["pos"] for s in systems]), dtype=torch.float64, device=dev)
+    diam = torch.tensor(np.stack([s["diam"] for s in systems]), dtype=torch.float64, device=dev)
+    out_v = torch.empty((F, G, 3, 3), dtype=torch.float64, device=dev)
+    eng = engine.Engine(dim=3, max_rcut=max_rcut, precision="fp32")
+    eng.set_box(L, [1, 1, 1])
+    eng.set_grid(grid)
+    eng.set_tuning(frames_per_launch=F)
+
+    def step():
+        eng.compute_frames(F, N, pos, diam, None, None, None, fields=("virial",), d_virial=out_v)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    gather_ms, n_contrib = 0.0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        st = eng.stats()
+        gather_ms += st["gather_ms"]
+        n_contrib += st["n_contrib"]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    fps = F * args.steps / elapsed
+    log("precision form: %d steps in %.3f s: %.2f frames/s" % (args.steps, elapsed, fps))
+    avg_launch_s = gather_ms / 1e3 / args.steps
+    flops = FLOP_U_3D * n_contrib / args.steps
+    achieved = flops / avg_launch_s / 1e12
+
+    # error vs the FP64 oracle on a sample of grid points of frame 0; the FP64
+    # GPU form on the same points for comparison
+    v32 = out_v[0].cpu().numpy()
+    e64 = engine.Engine(dim=3, max_rcut=max_rcut, precision="fp64")
+    e64.set_system(systems[0], grid)
+    v64 = e64.compute(fields=("virial",))["virial"]
+    rng = np.random.default_rng(7)
+    idx = np.sort(rng.choice(G, size=min(G, args.c3_sample), replace=False))
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", "0") or os.cpu_count() or 1), 16)
+    ref = oracle.ik_compute(systems[0], grid[idx], max_rcut=max_rcut, fields=("virial",), nthreads=threads)["virial"]
+    comps = {}
+    for a in range(3):
+        for b in range(a, 3):
+            scale = np.abs(ref[:, b, a]).max()
+            comps["%d%d" % (a, b)] = {
+                "fp32_form_max_abs_err_over_max_abs": float(np.abs(v32[idx, b, a] - ref[:, b, a]).max() / scale),
+                "fp64_form_max_abs_err_over_max_abs": float(np.abs(v64[idx, b, a] - ref[:, b, a]).max() / scale)}
+    worst = max(c["fp32_form_max_abs_err_over_max_abs"] for c in comps.values())
+    log("precision form: worst component error %.3e" % worst)
+    line = {"metric": METRIC, "value": round(fps, 3), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+            "data": "synthetic (seeded lattice+jitter polydisperse glass, 20 frames resident in HBM)",
+            "config": {"workload": "3D polydisperse12 N=131072, 20 frames, 64^3 grid, Octic r_cg=2.0, GL5, FP32 "
+                                   "forces and kernel weights, FP64 minimum image and accumulation, virial stress",
+                       "frames_per_step": F, "n_particles": N, "grid": "64^3", "parallelism": "single GPU"},
+            "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                         "kernel": "k_gather (field gather, FP32 form)",
+                         "compute": "FP32 VALU (published MI355X FP32 peak 157.3 TFLOP/s)",
+                         "avg_launch_ms": round(avg_launch_s * 1e3, 4), "flops_per_launch": flops},
+            "cpu_baseline": None,
+            "extra": {"contributions_per_s": n_contrib / elapsed, "contributions_per_frame": n_contrib / (F * args.steps),
+                      "error_vs_fp64_oracle": {"grid_points_sampled": int(len(idx)), "frame": 0,
+                                               "per_component": comps, "worst_fp32": worst}}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--config", type=int, default=2, choices=[2, 3],
+                    help="2: FP64 plain form, 2D N=65536 x 100 frames (default); 3: FP32 precision form, 3D")
+    ap.add_argument("--c3-sample", type=int, default=2048, help="grid points of the config-3 oracle check")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
@@ -127,6 +223,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+
+    if args.config == 3:
+        return precision_form(args)
 
     import torch
 
