@@ -184,10 +184,119 @@ def precision_form(args):
     print(json.dumps(line), flush=True)
 
 
+C5_N = (4096, 8192, 16384, 24576, 32768, 49152, 65536, 98304)
+
+
+def batched_sweep(args):
+    """Serving-style sweep: 1000 2D frames from 8 systems of uneven N (125
+    frames each, seeds 5000 + 1000 k + f), each on a 128 x 128 grid over its
+    own box, virial + kinetic.  One engine per system, launch shape tuned per
+    system by the autotuner; then the whole sweep runs (a) eagerly, system
+    after system, and (b) as one HIP graph holding every system's launches
+    (pgt_ik_enqueue_frames captured on one stream), replayed.  Both for the
+    FP64 plain form and the FP32 form."""
+    import torch
+
+    from pyglasstools_amd import engine, workload
+    from pyglasstools_amd.autotune import Autotuner
+
+    dev = torch.device("cuda:0")
+    FR, NG = 125, 128
+    log("generating %d systems x %d frames" % (len(C5_N), FR))
+    systems = []
+    for k, N in enumerate(C5_N):
+        frames = [workload.make_system(N, 2, 5000 + 1000 * k + f) for f in range(FR)]
+        L = frames[0]["L"]
+        d0 = float(frames[0]["diam"].max())
+        t = lambda key: torch.tensor(np.stack([fr[key] for fr in frames]), dtype=torch.float64, device=dev)
+        systems.append({"N": N, "L": L, "max_rcut": max(1.25 * d0, d0), "grid": workload.cartesian_grid_2d(L[0], NG),
+                        "pos": t("pos"), "diam": t("diam"), "mass": t("mass"), "vel": t("vel")})
+        del frames
+    G = NG * NG
+    results = {}
+    for prec in ("fp64", "fp32"):
+        tuner = Autotuner()
+        jobs = []
+        for sy in systems:
+            e = engine.Engine(dim=2, max_rcut=sy["max_rcut"], precision=prec)
+            e.set_box(sy["L"], [1, 1, 0])
+            e.set_grid(sy["grid"])
+            ov = torch.empty((FR, G, 2, 2), dtype=torch.float64, device=dev)
+            ok = torch.empty_like(ov)
+
+            def run(e=e, sy=sy, ov=ov, ok=ok):
+                e.compute_frames(FR, sy["N"], sy["pos"], sy["diam"], sy["mass"], sy["vel"], None,
+                                 fields=("virial", "kinetic"), d_virial=ov, d_kinetic=ok)
+
+            best, _ = tuner.tune(e, run, ("c5", prec, sy["N"]), FR, blocks=(128, 256), fpl=(25, FR),
+                                 sync=torch.cuda.synchronize, repeats=1)
+            jobs.append({"e": e, "sy": sy, "ov": ov, "ok": ok, "run": run, "tuning": best})
+        # (a) eager
+        for j in jobs:
+            j["run"]()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nc = 0
+        for _ in range(args.steps):
+            for j in jobs:
+                j["run"]()
+                nc += j["e"].stats()["n_contrib"]
+        torch.cuda.synchronize()
+        t_eager = (time.perf_counter() - t0) / args.steps
+        # (b) one graph over all systems
+        s = torch.cuda.Stream()
+        for j in jobs:
+            j["e"].set_stream(s.cuda_stream)
+        with torch.cuda.stream(s):
+            for j in jobs:
+                j["run"]()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            for j in jobs:
+                sy = j["sy"]
+                j["e"].enqueue_frames(FR, sy["N"], sy["pos"], sy["diam"], sy["mass"], sy["vel"], None,
+                                      fields=("virial", "kinetic"), d_virial=j["ov"], d_kinetic=j["ok"])
+        g.replay()
+        torch.cuda.synchronize()
+        for j in jobs:
+            j["e"].finish()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.replay()
+        torch.cuda.synchronize()
+        t_graph = (time.perf_counter() - t0) / args.steps
+        nc_graph = 0
+        for j in jobs:
+            j["e"].finish()  # also checks the bond capacity over every replay
+            nc_graph += j["e"].stats()["n_contrib"]
+            j["e"].set_stream(0)
+        nframes = FR * len(jobs)
+        results[prec] = {"frames_per_s_graph": nframes / t_graph, "frames_per_s_eager": nframes / t_eager,
+                         "contributions_per_s_graph": nc / args.steps / t_graph,
+                         "tuning": {str(j["sy"]["N"]): j["tuning"] for j in jobs}}
+        log("%s sweep: graph %.1f frames/s, eager %.1f frames/s" % (prec, nframes / t_graph, nframes / t_eager))
+        del jobs, g
+        torch.cuda.empty_cache()
+    line = {"metric": METRIC, "value": round(results["fp64"]["frames_per_s_graph"], 3), "unit": "frames/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": 1,
+            "ms_per_step": round(FR * len(C5_N) / results["fp64"]["frames_per_s_graph"] * 1e3, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded lattice+jitter polydisperse glasses, 8 systems x 125 frames resident in HBM)",
+            "config": {"workload": "2D polydisperse12, 8 systems N in %s x 125 frames, 128x128 grid each, Octic "
+                                   "r_cg=2.0, GL5, virial + kinetic; one HIP graph per sweep" % list(C5_N),
+                       "frames_per_step": FR * len(C5_N), "grid": "128x128", "parallelism": "single GPU"},
+            "roofline": None, "cpu_baseline": None,
+            "extra": {"fp64": results["fp64"], "fp32_form": results["fp32"],
+                      "fp32_over_fp64_graph": results["fp32"]["frames_per_s_graph"] / results["fp64"]["frames_per_s_graph"]}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", type=int, default=2, choices=[2, 3],
-                    help="2: FP64 plain form, 2D N=65536 x 100 frames (default); 3: FP32 precision form, 3D")
+    ap.add_argument("--config", type=int, default=2, choices=[2, 3, 5],
+                    help="2: FP64 plain form, 2D N=65536 x 100 frames (default); 3: FP32 precision form, 3D; "
+                         "5: batched sweep over 8 systems of uneven N in one HIP graph")
     ap.add_argument("--c3-sample", type=int, default=2048, help="grid points of the config-3 oracle check")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
@@ -226,6 +335,8 @@ def main():
 
     if args.config == 3:
         return precision_form(args)
+    if args.config == 5:
+        return batched_sweep(args)
 
     import torch
 

@@ -152,6 +152,28 @@ int pgt_ik_compute_frames(pgt_ik_engine *e, int nframes, int64_t n, const double
                           const double *d_disp, unsigned fields, double *d_out_virial,
                           double *d_out_kinetic, double *d_out_elastic);
 
+/* Enqueue a job like pgt_ik_compute_frames and return at once: no host
+ * synchronisation, no device allocation, no memset, no events, so a caller
+ * can capture the launches into a HIP graph (hipStreamBeginCapture on the
+ * engine's stream, see pgt_ik_set_stream) and replay them.  Needs an earlier
+ * pgt_ik_compute_frames on this engine with the same frame count, particle
+ * count and fields (it sizes the buffers and the bond-list capacity);
+ * otherwise PGT_ERR_STATE.  Check with pgt_ik_finish.
+ * No counterpart in the reference (whose compute runs one frame at a time on
+ * the CPU); it serves the batched, graph-replayed sweep over many frames and
+ * systems. */
+int pgt_ik_enqueue_frames(pgt_ik_engine *e, int nframes, int64_t n, const double *d_pos,
+                          const double *d_diam, const double *d_mass, const double *d_vel,
+                          const double *d_disp, unsigned fields, double *d_out_virial,
+                          double *d_out_kinetic, double *d_out_elastic);
+
+/* Wait for the engine's stream and check the work enqueued (or replayed)
+ * since the last finish: PGT_ERR_RANGE if a particle had more bonds than the
+ * capacity (those results are incomplete: run pgt_ik_compute_frames, which
+ * grows it) or an adaptive integral ran out of workspace.  Sets the counts in
+ * pgt_ik_get_stats.  No counterpart in the reference. */
+int pgt_ik_finish(pgt_ik_engine *e);
+
 /* Run the engine's work on the given hipStream_t (NULL: the engine's own). */
 int pgt_ik_set_stream(pgt_ik_engine *e, void *hip_stream);
 

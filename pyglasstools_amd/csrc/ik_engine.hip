@@ -49,6 +49,15 @@ int fail(int code, const std::string &msg)
             return fail(PGT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));           \
     } while (0)
 
+template <typename T>
+int fill_async(T *p, T v, size_t n, hipStream_t s)
+{
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // ------------------------------------------------------------------------
 // scalar / batched evaluators
 // ------------------------------------------------------------------------
@@ -424,8 +433,16 @@ size_t bond_entry_bytes(unsigned fields)
 // Launch the cell list, bond list and gather for nf frames (no host sync).
 int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos, const double *diam,
                const double *mass, const double *vel, const double *disp, unsigned fields, double *ovir,
-               double *okin, double *oela, bool z_flat, hipEvent_t ev_begin, hipEvent_t ev_gather, hipEvent_t ev_end)
+               double *okin, double *oela, bool z_flat, hipEvent_t ev_begin, hipEvent_t ev_gather, hipEvent_t ev_end,
+               bool no_alloc)
 {
+    // no_alloc: buffers must already be large enough (graph capture)
+    auto need = [&](auto &buf, size_t count) -> int {
+        if (!no_alloc) return buf.ensure(count);
+        if (buf.p && buf.n >= count) return 0;
+        return fail(PGT_ERR_STATE, "engine buffers are smaller than this job: run pgt_ik_compute_frames with the "
+                                   "same frame count and fields first");
+    };
     const int dim = e->prm.dim;
     const int D = (dim == 3 || !z_flat) ? 3 : 2;
     const int OD = dim;
@@ -445,18 +462,18 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     else if (OD == 2) ebytes = bond_entry_bytes<3, 2, double>(fields);
     else ebytes = fp32 ? bond_entry_bytes<3, 3, float>(fields) : bond_entry_bytes<3, 3, double>(fields);
     int rc;
-    if ((rc = e->d_pcell.ensure((size_t)nf * n)) || (rc = e->d_pslot.ensure((size_t)nf * n)) ||
-        (rc = e->d_order.ensure((size_t)nf * n)) || (rc = e->d_ccount.ensure((size_t)nf * g.ncell)) ||
-        (rc = e->d_cstart.ensure((size_t)nf * (g.ncell + 1))) || (rc = e->d_spos.ensure((size_t)nf * n * PS)) ||
-        (rc = e->d_sdiam.ensure((size_t)nf * n)) || (rc = e->d_bcount.ensure((size_t)nf * n)))
+    if ((rc = need(e->d_pcell, (size_t)nf * n)) || (rc = need(e->d_pslot, (size_t)nf * n)) ||
+        (rc = need(e->d_order, (size_t)nf * n)) || (rc = need(e->d_ccount, (size_t)nf * g.ncell)) ||
+        (rc = need(e->d_cstart, (size_t)nf * (g.ncell + 1))) || (rc = need(e->d_spos, (size_t)nf * n * PS)) ||
+        (rc = need(e->d_sdiam, (size_t)nf * n)) || (rc = need(e->d_bcount, (size_t)nf * n)))
         return rc;
-    if (kin && (rc = e->d_skin.ensure((size_t)nf * n * KN))) return rc;
-    if (ela && (rc = e->d_sdisp.ensure((size_t)nf * n * 3))) return rc;
+    if (kin && (rc = need(e->d_skin, (size_t)nf * n * KN))) return rc;
+    if (ela && (rc = need(e->d_sdisp, (size_t)nf * n * 3))) return rc;
     hipStream_t s = e->stream;
     if (ev_begin) HIPCHK(hipEventRecord(ev_begin, s));
     double *skin = kin ? e->d_skin.p : nullptr;
     double *sdisp = ela ? e->d_sdisp.p : nullptr;
-    HIPCHK(hipMemsetAsync(e->d_ccount.p, 0, sizeof(int) * (size_t)nf * g.ncell, s));
+    if ((rc = fill_async(e->d_ccount.p, 0, (size_t)nf * g.ncell, s))) return rc;
     const dim3 pgrid((unsigned)((n + 255) / 256), (unsigned)nf);
     hipLaunchKernelGGL(k_cell_count, pgrid, dim3(256), 0, s, g, n, pos, e->d_pcell.p, e->d_pslot.p, e->d_ccount.p);
     hipLaunchKernelGGL(k_cell_scan, dim3(nf), dim3(1024), 0, s, g.ncell, e->d_ccount.p, e->d_cstart.p);
@@ -500,14 +517,14 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     // per-wavefront statistics of this batch (see k_reduce_stats)
     const int nw_gather = (int)(((size_t)G + c.block - 1) / c.block) * nf * (c.block / 64);
     const int nw_bonds = pairs ? (int)(((size_t)n + kBondBlock - 1) / kBondBlock) * nf * (kBondBlock / 64) : 0;
-    if ((rc = e->d_wcounts.ensure(2 * (size_t)std::max(nw_gather, 1))) || (rc = e->d_wmax.ensure((size_t)std::max(nw_bonds, 1))))
+    if ((rc = need(e->d_wcounts, 2 * (size_t)std::max(nw_gather, 1))) || (rc = need(e->d_wmax, (size_t)std::max(nw_bonds, 1))))
         return rc;
     c.out.wave_counts = e->d_wcounts.p;
     if (pairs) {
         // entries beyond the capacity are counted, not written; the caller
         // checks d_maxcount after the job and re-runs with a larger capacity
         if (e->cap < 1) e->cap = D == 3 ? 32 : 16;
-        if ((rc = e->d_bonds.ensure((size_t)nf * n * (size_t)e->cap * ebytes))) return rc;
+        if ((rc = need(e->d_bonds, (size_t)nf * n * (size_t)e->cap * ebytes))) return rc;
         c.cap = e->cap;
         const PotParams P = make_pot(e->prm);
         if (D == 2) {
@@ -587,8 +604,8 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
     int mx = 0;
     for (int attempt = 0; attempt < 3; ++attempt) {
         // [0] largest bond count, [1] an adaptive integral ran out of workspace
-        HIPCHK(hipMemsetAsync(e->d_maxcount.p, 0, 2 * sizeof(int), e->stream));
-        HIPCHK(hipMemsetAsync(e->d_stats.p, 0, sizeof(unsigned long long) * 2, e->stream));
+        if ((rc = fill_async(e->d_maxcount.p, 0, 2, e->stream)) || (rc = fill_async(e->d_stats.p, 0ull, 2, e->stream)))
+            return rc;
         for (int b = 0; b < nb; ++b) {
             const int f0 = b * fpl;
             const int nf = std::min(fpl, fs.nframes - f0);
@@ -597,7 +614,7 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
                             fs.vel ? fs.vel + po * 3 : nullptr, fs.disp ? fs.disp + po * 3 : nullptr, fields,
                             fs.vir ? fs.vir + (size_t)f0 * fsz : nullptr, fs.kin ? fs.kin + (size_t)f0 * fsz : nullptr,
                             fs.ela ? fs.ela + (size_t)f0 * fsz : nullptr, fs.z_flat, e->ev[3 * b],
-                            e->ev[3 * b + 1], e->ev[3 * b + 2]);
+                            e->ev[3 * b + 1], e->ev[3 * b + 2], false);
             if (rc) return rc;
         }
         int flags[2] = {0, 0};
@@ -629,6 +646,34 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
     e->stats.total_ms = tot;
     e->stats.bond_capacity = e->cap;
     e->stats.max_bonds = mx;
+    return 0;
+}
+
+// Enqueue every batch of a job on the engine's stream and return: no host
+// synchronisation, no allocation, no events, no memset, so the launches can
+// be captured into a HIP graph.  Needs buffers and a bond capacity sized by an
+// earlier run_job of the same shape; an overflow of that capacity during the
+// enqueued work shows up in pgt_ik_finish.
+int enqueue_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
+{
+    const int OD = e->prm.dim;
+    const size_t fsz = (size_t)e->G * OD * OD;
+    const int fpl = std::max(1, e->frames_per_launch);
+    const int nb = (fs.nframes + fpl - 1) / fpl;
+    if (e->cap < 1 || !e->d_maxcount.p || e->d_maxcount.n < 2 || !e->d_stats.p)
+        return fail(PGT_ERR_STATE, "run pgt_ik_compute_frames of the same shape before enqueueing");
+    for (int b = 0; b < nb; ++b) {
+        const int f0 = b * fpl;
+        const int nf = std::min(fpl, fs.nframes - f0);
+        const size_t po = (size_t)f0 * fs.n;
+        const int rc = run_frames(e, nf, fs.n, fs.pos + po * 3, fs.diam + po, fs.mass ? fs.mass + po : nullptr,
+                                  fs.vel ? fs.vel + po * 3 : nullptr, fs.disp ? fs.disp + po * 3 : nullptr, fields,
+                                  fs.vir ? fs.vir + (size_t)f0 * fsz : nullptr,
+                                  fs.kin ? fs.kin + (size_t)f0 * fsz : nullptr,
+                                  fs.ela ? fs.ela + (size_t)f0 * fsz : nullptr, fs.z_flat, nullptr, nullptr, nullptr,
+                                  true);
+        if (rc) return rc;
+    }
     return 0;
 }
 
@@ -779,6 +824,71 @@ int pgt_ik_set_grid(pgt_ik_engine *e, int64_t g, const double *grid)
     e->grid_z_flat = flat;
     e->G = g;
     e->have_grid = true;
+    return PGT_OK;
+}
+
+int pgt_ik_enqueue_frames(pgt_ik_engine *e, int nframes, int64_t n, const double *d_pos, const double *d_diam,
+                          const double *d_mass, const double *d_vel, const double *d_disp, unsigned fields,
+                          double *d_out_virial, double *d_out_kinetic, double *d_out_elastic)
+{
+    if (int rc = check_engine(e)) return rc;
+    if (!e->box_set || !e->have_grid) return fail(PGT_ERR_STATE, "set_box and set_grid must precede enqueue_frames");
+    if (nframes < 0 || n < 0 || n > (int64_t)1 << 30) return fail(PGT_ERR_ARG, "bad frame or particle count");
+    if (fields == 0 || (fields & ~7u)) return fail(PGT_ERR_ARG, "bad field mask");
+    if (((fields & PGT_FIELD_VIRIAL) && !d_out_virial) || ((fields & PGT_FIELD_KINETIC) && !d_out_kinetic) ||
+        ((fields & PGT_FIELD_ELASTIC) && !d_out_elastic))
+        return fail(PGT_ERR_ARG, "null output for a selected field");
+    if (n > 0 && (!d_pos || !d_diam)) return fail(PGT_ERR_ARG, "positions and diameters are required");
+    if (nframes == 0 || e->G == 0) return PGT_OK;
+    const int OD = e->prm.dim;
+    const size_t fsz = (size_t)e->G * OD * OD;
+    if (n == 0) {
+        const unsigned bits[3] = {PGT_FIELD_VIRIAL, PGT_FIELD_KINETIC, PGT_FIELD_ELASTIC};
+        double *outs[3] = {d_out_virial, d_out_kinetic, d_out_elastic};
+        for (int k = 0; k < 3; ++k)
+            if (fields & bits[k])
+                if (int rc = fill_async(outs[k], 0.0, fsz * nframes, e->stream)) return rc;
+        return PGT_OK;
+    }
+    FrameSet fs{};
+    fs.nframes = nframes;
+    fs.n = n;
+    fs.pos = d_pos;
+    fs.diam = d_diam;
+    fs.mass = d_mass;
+    fs.vel = d_vel;
+    fs.disp = d_disp;
+    fs.vir = (fields & PGT_FIELD_VIRIAL) ? d_out_virial : nullptr;
+    fs.kin = (fields & PGT_FIELD_KINETIC) ? d_out_kinetic : nullptr;
+    fs.ela = (fields & PGT_FIELD_ELASTIC) ? d_out_elastic : nullptr;
+    fs.z_flat = e->grid_z_flat;
+    return enqueue_job(e, fs, fields);
+}
+
+int pgt_ik_finish(pgt_ik_engine *e)
+{
+    if (int rc = check_engine(e)) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (!e->d_maxcount.p || !e->d_stats.p) return PGT_OK;
+    int flags[2] = {0, 0};
+    unsigned long long h[2] = {0, 0};
+    HIPCHK(hipMemcpy(flags, e->d_maxcount.p, sizeof(flags), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h, e->d_stats.p, sizeof(h), hipMemcpyDeviceToHost));
+    int rc = fill_async(e->d_maxcount.p, 0, 2, e->stream);
+    if (!rc) rc = fill_async(e->d_stats.p, 0ull, 2, e->stream);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->stats.n_ball = (int64_t)h[0];
+    e->stats.n_contrib = (int64_t)h[1];
+    e->stats.max_bonds = flags[0];
+    e->stats.bond_capacity = e->cap;
+    if (flags[1])
+        return fail(PGT_ERR_RANGE, "an adaptive bond integral needed more than " + std::to_string(kQagMax) +
+                                       " intervals (the GPU rule's workspace)");
+    if (flags[0] > e->cap)
+        return fail(PGT_ERR_RANGE, "a particle had more bonds than the capacity sized by the last "
+                                   "pgt_ik_compute_frames: the enqueued results are incomplete; run "
+                                   "pgt_ik_compute_frames on these frames to grow it");
     return PGT_OK;
 }
 

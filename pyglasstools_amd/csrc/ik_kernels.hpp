@@ -215,6 +215,14 @@ __global__ void k_cell_rank(int n, int ncell, const int *__restrict__ pcell, con
         for (int a = 0; a < 3; ++a) sdisp[fk * 3 + a] = disp ? disp[fi * 3 + a] : 0.0;
 }
 
+// Clears that a captured HIP graph can hold (a memset node is avoided there).
+template <typename T>
+__global__ void k_fill(T *__restrict__ p, T v, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // Fold the per-wavefront statistics of one batch into the job totals: sums
 // of (balls, contributions) into stats[0..1], the largest bond count into
 // maxcount[0].  Each workgroup reduces a stretch of the per-wavefront arrays

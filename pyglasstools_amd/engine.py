@@ -21,7 +21,8 @@ PREC = {"fp64": 0, "fp32": 1}
 # every symbol include/pgt_ik.h declares
 EXPORTED = [
     "pgt_last_error", "pgt_device_count", "pgt_ik_create", "pgt_ik_destroy", "pgt_ik_set_box",
-    "pgt_ik_set_particles", "pgt_ik_set_grid", "pgt_ik_compute", "pgt_ik_compute_frames", "pgt_ik_set_stream",
+    "pgt_ik_set_particles", "pgt_ik_set_grid", "pgt_ik_compute", "pgt_ik_compute_frames", "pgt_ik_enqueue_frames",
+    "pgt_ik_finish", "pgt_ik_set_stream",
     "pgt_ik_set_tuning", "pgt_ik_get_stats", "pgt_eval_pair_force", "pgt_eval_cgfunc",
 ]
 
@@ -81,6 +82,8 @@ def lib():
         L.pgt_ik_set_grid.argtypes = [vp, i64, vp]
         L.pgt_ik_compute.argtypes = [vp, ctypes.c_uint, vp, vp, vp]
         L.pgt_ik_compute_frames.argtypes = [vp, i, i64, vp, vp, vp, vp, vp, ctypes.c_uint, vp, vp, vp]
+        L.pgt_ik_enqueue_frames.argtypes = [vp, i, i64, vp, vp, vp, vp, vp, ctypes.c_uint, vp, vp, vp]
+        L.pgt_ik_finish.argtypes = [vp]
         L.pgt_ik_set_stream.argtypes = [vp, vp]
         L.pgt_ik_set_tuning.argtypes = [vp, i, i]
         L.pgt_ik_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
@@ -228,6 +231,20 @@ class Engine(object):
     def set_stream(self, stream_handle):
         _check(lib().pgt_ik_set_stream(self.h, ctypes.c_void_p(stream_handle) if stream_handle else None),
                "pgt_ik_set_stream")
+
+    def enqueue_frames(self, nframes, n, d_pos, d_diam, d_mass=None, d_vel=None, d_disp=None, fields=("virial",),
+                       d_virial=None, d_kinetic=None, d_elastic=None):
+        """compute_frames without host synchronisation or allocation (graph
+        capture); needs an earlier compute_frames of the same shape; check
+        with finish()."""
+        mask = (FIELD_VIRIAL if "virial" in fields else 0) | (FIELD_KINETIC if "kinetic" in fields else 0) | \
+               (FIELD_ELASTIC if "elastic" in fields else 0)
+        _check(lib().pgt_ik_enqueue_frames(self.h, nframes, n, _ptr(d_pos), _ptr(d_diam), _ptr(d_mass), _ptr(d_vel),
+                                           _ptr(d_disp), mask, _ptr(d_virial), _ptr(d_kinetic), _ptr(d_elastic)),
+               "pgt_ik_enqueue_frames")
+
+    def finish(self):
+        _check(lib().pgt_ik_finish(self.h), "pgt_ik_finish")
 
     def set_tuning(self, threads_per_block=0, frames_per_launch=0):
         _check(lib().pgt_ik_set_tuning(self.h, threads_per_block, frames_per_launch), "pgt_ik_set_tuning")

@@ -361,3 +361,56 @@ def test_adaptive_fields_match_oracle(gpu, cg):
     # each bond integral is good to max(epsabs, epsrel |I|) whatever path the
     # bisection takes, so the field is good to that relative to its scale
     assert_parity(got, ref, st, ("virial", "kinetic"), tol=1e-8)
+
+
+def test_graph_captured_batches_match_eager(gpu):
+    """Two systems of different N, each a few frames, captured into one HIP
+    graph through enqueue_frames (no sync, no allocation, no memset nodes) and
+    replayed three times with eager work in between: every replay must give
+    the eager compute_frames result bit for bit."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    jobs = []
+    for N, nf, seed0 in ((2000, 3, 900), (3000, 2, 950)):
+        frames = [workload.make_system(N, 2, seed0 + f) for f in range(nf)]
+        L = frames[0]["L"]
+        grid = workload.cartesian_grid_2d(L[0], 24)
+        e = engine.Engine(dim=2, max_rcut=max(1.25 * 1.62, 1.62))
+        e.set_box(L, [1, 1, 0])
+        e.set_grid(grid)
+        e.set_tuning(frames_per_launch=nf)
+        t = lambda k: torch.tensor(np.stack([f[k] for f in frames]), dtype=torch.float64, device=dev)
+        pos, diam, vel = t("pos"), t("diam"), t("vel")
+        ov = torch.zeros((nf, len(grid), 2, 2), dtype=torch.float64, device=dev)
+        ok = torch.zeros_like(ov)
+        e.compute_frames(nf, N, pos, diam, None, vel, None, fields=("virial", "kinetic"), d_virial=ov, d_kinetic=ok)
+        torch.cuda.synchronize()
+        jobs.append(dict(e=e, N=N, nf=nf, pos=pos, diam=diam, vel=vel, ov=ov, ok=ok,
+                         ref_v=ov.clone(), ref_k=ok.clone()))
+    s = torch.cuda.Stream()
+    for j in jobs:
+        j["e"].set_stream(s.cuda_stream)
+        j["ov"].zero_()
+        j["ok"].zero_()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        for j in jobs:
+            j["e"].enqueue_frames(j["nf"], j["N"], j["pos"], j["diam"], None, j["vel"], None,
+                                  fields=("virial", "kinetic"), d_virial=j["ov"], d_kinetic=j["ok"])
+    for rep in range(3):
+        for j in jobs:
+            j["ov"].fill_(-1.0)
+            j["ok"].fill_(-1.0)
+        junk = torch.randn(1 << 16, device=dev).sum()  # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for j in jobs:
+            j["e"].finish()
+            assert torch.equal(j["ov"], j["ref_v"]), "replay %d virial" % rep
+            assert torch.equal(j["ok"], j["ref_k"]), "replay %d kinetic" % rep
+        del junk
+    for j in jobs:
+        j["e"].set_stream(0)
