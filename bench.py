@@ -310,6 +310,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--block", type=int, default=0, help="gather threads per workgroup (0: autotune)")
     ap.add_argument("--fpl", type=int, default=0, help="frames per launch (0: autotune)")
+    ap.add_argument("--pl", type=int, default=0, help="grid points per gather lane (0: autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (reference dispatches for HBM byte counters) and exit")
     args = ap.parse_args()
@@ -373,11 +374,11 @@ def main():
                            d_kinetic=out_k)
 
     tuned, trials = None, []
-    if args.block and args.fpl:
-        eng.set_tuning(args.block, args.fpl)
-        tuned = {"threads_per_block": args.block, "frames_per_launch": args.fpl}
+    if args.block and args.fpl and args.pl:
+        eng.set_tuning(args.block, args.fpl, args.pl)
+        tuned = {"threads_per_block": args.block, "frames_per_launch": args.fpl, "points_per_lane": args.pl}
     else:
-        log("autotuning lanes per grid point x frames per launch")
+        log("autotuning threads per workgroup x frames per launch x points per lane")
         tuned, trials = Autotuner().tune(eng, step, ("c2", N, G, F, args.precision), F, sync=torch.cuda.synchronize)
         log("tuned: %s" % tuned)
 

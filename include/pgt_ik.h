@@ -177,9 +177,11 @@ int pgt_ik_finish(pgt_ik_engine *e);
 /* Run the engine's work on the given hipStream_t (NULL: the engine's own). */
 int pgt_ik_set_stream(pgt_ik_engine *e, void *hip_stream);
 
-/* Threads per workgroup of the field-gather kernel (128 or 256) and frames
- * per launch; 0 keeps the current choice.  Used by the autotuner. */
-int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_launch);
+/* Launch shape of the field gather: threads per workgroup (128 or 256),
+ * frames per launch, and grid points per lane (1, 2 or 4: neighbouring
+ * points that share one read of each particle and bond entry); 0 keeps the
+ * current choice.  Used by the autotuner.  No counterpart in the reference. */
+int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_launch, int points_per_lane);
 
 int pgt_ik_get_stats(pgt_ik_engine *e, pgt_ik_stats *out);
 

@@ -243,6 +243,7 @@ struct pgt_ik_engine {
     int cap = 0;
     int block = 256;
     int frames_per_launch = 8;
+    int points_per_lane = 1;  // grid points per gather lane (Octic 5-node rule; 1 elsewhere)
     pgt_ik_stats stats{};
 };
 
@@ -338,7 +339,9 @@ struct LaunchCtx {
     int n, G, nf;
     int cap;
     int block;
-    bool pack;  // LDS list entries as index | bond count << 26
+    int pl;     // grid points per gather lane
+    bool pack;  // LDS list entries packed into one int (see k_gather)
+    mutable int pl_launched;  // points per lane of the gather actually launched
     const double *grid;
     const int *perm;
     GLRule R;
@@ -352,29 +355,47 @@ struct LaunchCtx {
     hipStream_t stream;
 };
 
-template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, typename ER, int BLK>
+// Gather points per lane in use: the tuned value where launch_gather_c
+// picks the unrolled Octic 5-node rule, 1 elsewhere.
+inline int gather_pl(int tuned, int cg, int nq, int D, int OD)
+{
+    return (cg == 0 && nq == 5 && !(D == 3 && OD == 2)) ? tuned : 1;
+}
+
+template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, typename ER, int BLK, int PL>
 void launch_gather_t(const LaunchCtx &c)
 {
     // per-lane list length: ~2x the mean number of particles in a ball at
-    // unit density (12.6 in 2D, 33.5 in 3D for r_cg = 2); longer lists are
-    // processed in pieces
-    constexpr int MAXL = D == 2 ? 24 : 48;
-    dim3 grid((unsigned)((c.G + BLK - 1) / BLK), (unsigned)c.nf);
+    // unit density (12.6 in 2D, 33.5 in 3D for r_cg = 2), more for the union
+    // of several points' balls; longer lists are processed in pieces
+    constexpr int MAXL = (D == 2 ? 24 : 48) * (PL == 1 ? 3 : PL == 2 ? 4 : 5) / 3;
+    dim3 grid((unsigned)((c.G + BLK * PL - 1) / (BLK * PL)), (unsigned)c.nf);
+    c.pl_launched = PL;
     if (c.pack)
-        hipLaunchKernelGGL((k_gather<D, OD, CG, NQ, VIR, ELA, ER, BLK, MAXL, true>), grid, dim3(BLK), 0, c.stream,
+        hipLaunchKernelGGL((k_gather<D, OD, CG, NQ, VIR, ELA, ER, BLK, MAXL, true, PL>), grid, dim3(BLK), 0, c.stream,
                            c.g, c.n, c.G, c.grid, c.perm, c.R, c.gld, c.cgnorm, c.cstart, c.spos, c.skin, c.cap,
                            c.bcount, (const ER *)c.bonds, c.out);
     else
-        hipLaunchKernelGGL((k_gather<D, OD, CG, NQ, VIR, ELA, ER, BLK, MAXL, false>), grid, dim3(BLK), 0, c.stream,
-                           c.g, c.n, c.G, c.grid, c.perm, c.R, c.gld, c.cgnorm, c.cstart, c.spos, c.skin, c.cap,
-                           c.bcount, (const ER *)c.bonds, c.out);
+        hipLaunchKernelGGL((k_gather<D, OD, CG, NQ, VIR, ELA, ER, BLK, MAXL, false, PL>), grid, dim3(BLK), 0,
+                           c.stream, c.g, c.n, c.G, c.grid, c.perm, c.R, c.gld, c.cgnorm, c.cstart, c.spos, c.skin,
+                           c.cap, c.bcount, (const ER *)c.bonds, c.out);
+}
+
+template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, typename ER, int BLK>
+void launch_gather_p(const LaunchCtx &c)
+{
+    if constexpr (CG == 0 && NQ == 5) {
+        if (c.pl == 4) return launch_gather_t<D, OD, CG, NQ, VIR, ELA, ER, BLK, 4>(c);
+        if (c.pl == 2) return launch_gather_t<D, OD, CG, NQ, VIR, ELA, ER, BLK, 2>(c);
+    }
+    launch_gather_t<D, OD, CG, NQ, VIR, ELA, ER, BLK, 1>(c);
 }
 
 template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, typename ER>
 void launch_gather_b(const LaunchCtx &c)
 {
-    if (c.block == 128) launch_gather_t<D, OD, CG, NQ, VIR, ELA, ER, 128>(c);
-    else launch_gather_t<D, OD, CG, NQ, VIR, ELA, ER, 256>(c);
+    if (c.block == 128) launch_gather_p<D, OD, CG, NQ, VIR, ELA, ER, 128>(c);
+    else launch_gather_p<D, OD, CG, NQ, VIR, ELA, ER, 256>(c);
 }
 
 template <int D, int OD, int CG, int NQ, typename ER>
@@ -474,19 +495,21 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     double *skin = kin ? e->d_skin.p : nullptr;
     double *sdisp = ela ? e->d_sdisp.p : nullptr;
     if ((rc = fill_async(e->d_ccount.p, 0, (size_t)nf * g.ncell, s))) return rc;
-    const dim3 pgrid((unsigned)((n + 255) / 256), (unsigned)nf);
-    hipLaunchKernelGGL(k_cell_count, pgrid, dim3(256), 0, s, g, n, pos, e->d_pcell.p, e->d_pslot.p, e->d_ccount.p);
+    // frame-per-XCD layout of the cell-list kernels (frame_block)
+    const dim3 pgrid((unsigned)((n + 255) / 256), (unsigned)frame_rows(nf));
+    hipLaunchKernelGGL(k_cell_count, pgrid, dim3(256), 0, s, g, n, nf, pos, e->d_pcell.p, e->d_pslot.p,
+                       e->d_ccount.p);
     hipLaunchKernelGGL(k_cell_scan, dim3(nf), dim3(1024), 0, s, g.ncell, e->d_ccount.p, e->d_cstart.p);
-    hipLaunchKernelGGL(k_cell_fill, pgrid, dim3(256), 0, s, n, g.ncell, e->d_pcell.p, e->d_pslot.p,
+    hipLaunchKernelGGL(k_cell_fill, pgrid, dim3(256), 0, s, n, nf, g.ncell, e->d_pcell.p, e->d_pslot.p,
                        e->d_cstart.p, e->d_order.p);
     if (D == 2)
-        hipLaunchKernelGGL((k_cell_rank<2, 2>), pgrid, dim3(256), 0, s, n, g.ncell, e->d_pcell.p, e->d_cstart.p,
+        hipLaunchKernelGGL((k_cell_rank<2, 2>), pgrid, dim3(256), 0, s, n, nf, g.ncell, e->d_pcell.p, e->d_cstart.p,
                            e->d_order.p, pos, diam, mass, vel, disp, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
     else if (OD == 2)
-        hipLaunchKernelGGL((k_cell_rank<3, 2>), pgrid, dim3(256), 0, s, n, g.ncell, e->d_pcell.p, e->d_cstart.p,
+        hipLaunchKernelGGL((k_cell_rank<3, 2>), pgrid, dim3(256), 0, s, n, nf, g.ncell, e->d_pcell.p, e->d_cstart.p,
                            e->d_order.p, pos, diam, mass, vel, disp, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
     else
-        hipLaunchKernelGGL((k_cell_rank<3, 3>), pgrid, dim3(256), 0, s, n, g.ncell, e->d_pcell.p, e->d_cstart.p,
+        hipLaunchKernelGGL((k_cell_rank<3, 3>), pgrid, dim3(256), 0, s, n, nf, g.ncell, e->d_pcell.p, e->d_cstart.p,
                            e->d_order.p, pos, diam, mass, vel, disp, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
     HIPCHK(hipGetLastError());
 
@@ -515,9 +538,13 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     c.stream = s;
 
     // per-wavefront statistics of this batch (see k_reduce_stats)
-    const int nw_gather = (int)(((size_t)G + c.block - 1) / c.block) * nf * (c.block / 64);
+    c.pl = adaptive ? 1 : gather_pl(e->points_per_lane, e->prm.cg_kind, (int)e->gl_s.size(), D, OD);
+    const size_t lanes_g = ((size_t)G + c.pl - 1) / c.pl;  // gather lanes per frame
+    const int nw_gather = (int)((lanes_g + c.block - 1) / c.block) * nf * (c.block / 64);
+    // sized for one point per lane, the most wavefronts any shape launches
+    const int nw_gather_max = (int)(((size_t)G + c.block - 1) / c.block) * nf * (c.block / 64);
     const int nw_bonds = pairs ? (int)(((size_t)n + kBondBlock - 1) / kBondBlock) * nf * (kBondBlock / 64) : 0;
-    if ((rc = need(e->d_wcounts, 2 * (size_t)std::max(nw_gather, 1))) || (rc = need(e->d_wmax, (size_t)std::max(nw_bonds, 1))))
+    if ((rc = need(e->d_wcounts, 2 * (size_t)std::max(nw_gather_max, 1))) || (rc = need(e->d_wmax, (size_t)std::max(nw_bonds, 1))))
         return rc;
     c.out.wave_counts = e->d_wcounts.p;
     if (pairs) {
@@ -542,7 +569,7 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     }
     c.bcount = e->d_bcount.p;
     c.bonds = e->d_bonds.p;
-    c.pack = n < (1 << 26) && c.cap <= 63;
+    c.pack = n < (1 << (c.pl > 1 ? 22 : 26)) && c.cap <= 63;
     if (ev_gather) HIPCHK(hipEventRecord(ev_gather, s));
     const int cg = e->prm.cg_kind;
     if (adaptive) {
@@ -574,6 +601,8 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
         else launch_gather_c<3, 3, double>(c, fields, cg);
     }
     HIPCHK(hipGetLastError());
+    if (!adaptive && c.pl_launched != c.pl)
+        return fail(PGT_ERR_STATE, "gather launched with an unexpected number of points per lane");
     if (ev_end) HIPCHK(hipEventRecord(ev_end, s));
     const int nrb = std::max(1, std::min(256, (std::max(nw_gather, nw_bonds) + 4095) / 4096));
     hipLaunchKernelGGL(k_reduce_stats, dim3(nrb), dim3(1024), 0, s, nw_gather, e->d_wcounts.p, nw_bonds, e->d_wmax.p,
@@ -900,7 +929,7 @@ int pgt_ik_set_stream(pgt_ik_engine *e, void *hip_stream)
     return PGT_OK;
 }
 
-int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_launch)
+int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_launch, int points_per_lane)
 {
     if (int rc = check_engine(e)) return rc;
     if (threads_per_block) {
@@ -911,6 +940,11 @@ int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_la
     if (frames_per_launch) {
         if (frames_per_launch < 1 || frames_per_launch > 65535) return fail(PGT_ERR_ARG, "bad frames per launch");
         e->frames_per_launch = frames_per_launch;
+    }
+    if (points_per_lane) {
+        if (points_per_lane != 1 && points_per_lane != 2 && points_per_lane != 4)
+            return fail(PGT_ERR_ARG, "grid points per lane must be 1, 2 or 4");
+        e->points_per_lane = points_per_lane;
     }
     return PGT_OK;
 }

@@ -65,22 +65,63 @@ __device__ __forceinline__ void load_entry(const ER *__restrict__ p, ER (&v)[ES]
     }
 }
 
-// Field gather, one lane per grid point (grid points taken in the order of
-// `perm`, which puts nearby points in the same wavefront), reading particles
-// and bond entries straight from global memory.
+// Cells along one axis of the cell grid that cover cells c_j - 1 .. c_j + 1
+// of PL points (point 0 always valid, the others where v[j]): `lo` (in
+// [0, nc)) and `cnt`, taken periodically; the whole axis when cnt would reach
+// nc (so no cell is listed twice).  Axes with fewer than 3 cells are
+// non-periodic or too short for a window and are taken whole.
+template <int PL>
+__device__ __forceinline__ void axis_window(int nc, const int (&c)[PL], const bool (&v)[PL], int &lo, int &cnt)
+{
+    if (nc < 3) {
+        lo = 0;
+        cnt = nc;
+        return;
+    }
+    int dmin = 0, dmax = 0;
+#pragma unroll
+    for (int j = 1; j < PL; ++j) {
+        if (!v[j]) continue;
+        int d = c[j] - c[0];
+        if (2 * d > nc) d -= nc;
+        else if (2 * d <= -nc) d += nc;
+        dmin = min(dmin, d);
+        dmax = max(dmax, d);
+    }
+    cnt = dmax - dmin + 3;
+    if (cnt >= nc) {
+        lo = 0;
+        cnt = nc;
+        return;
+    }
+    lo = c[0] + dmin - 1;
+    if (lo < 0) lo += nc;
+    else if (lo >= nc) lo -= nc;
+}
+
+// Field gather.  Each lane takes PL consecutive grid points in the order of
+// `perm` (Morton order, so 2 points are x-neighbours and 4 a 2x2 block on a
+// regular grid) and reads particles and bond entries straight from global
+// memory.  A particle inside the coarse-graining ball of several of the
+// lane's points is read once, and each of its bond entries once, for all of
+// them: on a regular 2D grid with spacing r_cg/2 a lane of 4 points reads a
+// particle for 2.3 points on average, which divides the scattered loads that
+// bound this kernel.
 //
-// Pass A scans the cells around the grid point and compacts the particles
-// inside the coarse-graining ball, with their bond counts, into a per-lane list in LDS
-// (one int per entry, index | count << 26, when PACK is set: fewer than 2^26
-// particles and at most 63 bonds each; otherwise a second array holds the
-// counts).  The list is then sorted by bond count (largest first), so that at
-// every step of pass B the lanes of a wavefront hold particles with similar
-// bond counts and the bond loop wastes few lanes.  Pass B adds each
-// particle's kinetic term and integrates its bonds; the next bond entry and
+// Pass A scans the cells around the lane's points and compacts the particles
+// inside at least one ball, with their bond counts and the mask of the points
+// whose ball holds them, into a per-lane list in LDS.  With PACK one int per
+// entry holds index | count << IB | mask << (IB + 6) (IB = 26 for one point
+// per lane, 22 otherwise: fewer than 2^IB particles and at most 63 bonds
+// each); otherwise separate arrays hold the counts and masks.  The list is
+// then sorted by bond count (largest first), so that at every step of pass B
+// the lanes of a wavefront hold particles with similar bond counts and the
+// bond loop wastes few lanes.  Pass B adds each particle's kinetic term and
+// integrates its bonds for the points of its mask; the next bond entries and
 // the next particle's data are loaded before the current ones are used, so
 // memory latency overlaps the arithmetic.  A full list is processed early.
-// All sums are FP64 in a fixed order.
-template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, typename ER, int BLK, int MAXL, bool PACK>
+// All sums are FP64, per grid point in list order.
+template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, typename ER, int BLK, int MAXL, bool PACK, int PL>
 __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const double *__restrict__ grid,
                                                 const int *__restrict__ perm, GLRule R,
                                                 const double *__restrict__ gld, double cgnorm,
@@ -89,6 +130,7 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                                                 const int *__restrict__ bcount, const ER *__restrict__ bonds,
                                                 GatherOut out)
 {
+    static_assert(PL == 1 || PL == 2 || PL == 4, "1, 2 or 4 grid points per lane");
     constexpr int PS = (D == 2) ? 2 : 4;
     constexpr int KN = OD * (OD + 1) / 2;
     constexpr bool PAIRS = VIR || ELA;
@@ -96,20 +138,27 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
     constexpr int ES = LY::ES;    // values per entry
     constexpr int GRP = LY::GRP;  // entries per read group
     constexpr int GV = LY::GV;    // values per read group
+    constexpr int IB = PL > 1 ? 22 : 26;  // index bits of a packed list entry
     __shared__ int lp[MAXL][BLK];
     __shared__ unsigned short lnb[(PAIRS && !PACK) ? MAXL : 1][BLK];
+    __shared__ unsigned char lmk[(PL > 1 && !PACK) ? MAXL : 1][BLK];
 
     const int f = blockIdx.y;
     const int lane = threadIdx.x;
-    // list entry k of this lane: particle index and bond count
-    auto getp = [&](int k) -> int { return PACK ? (lp[k][lane] & 0x3FFFFFF) : lp[k][lane]; };
+    // list entry k of this lane: particle index, bond count, point mask
+    auto getp = [&](int k) -> int { return PACK ? (lp[k][lane] & ((1 << IB) - 1)) : lp[k][lane]; };
     auto getnb = [&](int k) -> int {
         if constexpr (!PAIRS) return 0;
-        else if constexpr (PACK) return (int)((unsigned)lp[k][lane] >> 26);
+        else if constexpr (PACK) return (int)(((unsigned)lp[k][lane] >> IB) & 63u);
         else return lnb[k][lane];
     };
-    const int t = xcd_tile(blockIdx.x, gridDim.x) * BLK + lane;
-    const bool live = t < G;
+    auto getm = [&](int k) -> int {
+        if constexpr (PL == 1) return 1;
+        else if constexpr (PACK) return (int)((unsigned)lp[k][lane] >> (IB + 6));
+        else return lmk[k][lane];
+    };
+    const int t0 = (xcd_tile(blockIdx.x, gridDim.x) * BLK + lane) * PL;  // first point slot of the lane
+    const bool live = t0 < G;
     const bool KIN = skin != nullptr;
     const double *sp = spos + (size_t)f * n * PS;
     const int *st = cstart + (size_t)f * (g.ncell + 1);
@@ -117,24 +166,38 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
     const ER *bd = bonds + (size_t)f * n * (size_t)cap * ES;
     const double *sk = KIN ? skin + (size_t)f * n * KN : nullptr;
 
-    double av[KN], ae[KN], ak[KN];
+    double av[PL][KN], ae[PL][KN], ak[PL][KN];
 #pragma unroll
-    for (int k = 0; k < KN; ++k) { av[k] = 0.0; ae[k] = 0.0; ak[k] = 0.0; }
+    for (int j = 0; j < PL; ++j)
+#pragma unroll
+        for (int k = 0; k < KN; ++k) { av[j][k] = 0.0; ae[j][k] = 0.0; ak[j][k] = 0.0; }
     unsigned long long nball = 0, ncontrib = 0;
-    const int gp = live ? perm[t] : 0;
+    int gp[PL];
+    bool pv[PL];
+#pragma unroll
+    for (int j = 0; j < PL; ++j) {
+        pv[j] = t0 + j < G;
+        gp[j] = pv[j] ? perm[t0 + j] : 0;
+    }
 
     if (live) {
-        const double x0 = grid[(size_t)gp * 3], x1 = grid[(size_t)gp * 3 + 1];
-        const double x2 = (D == 3) ? grid[(size_t)gp * 3 + 2] : 0.0;
+        // coordinates of the lane's points; a missing point takes point 0's
+        // (its mask bit is never set)
+        double xg[PL][3];
+#pragma unroll
+        for (int j = 0; j < PL; ++j) {
+            const int q = pv[j] ? gp[j] : gp[0];
+            xg[j][0] = grid[(size_t)q * 3];
+            xg[j][1] = grid[(size_t)q * 3 + 1];
+            xg[j][2] = (D == 3) ? grid[(size_t)q * 3 + 2] : 0.0;
+        }
         const double s2 = 2.0 * g.inv_rc;
 
         // integrate the entry at offset O (a std::integral_constant) of a
-        // read group
-        auto integrate = [&](const ER (&G)[GV], auto O, ER ta0, ER ta1, ER ta2, ER A) {
+        // read group for the points of mask m
+        auto integrate = [&](const ER (&Gv)[GV], auto O, int m, const ER (&ta)[PL][3], const ER (&Aj)[PL]) {
             constexpr int o = decltype(O)::value;
-            const ER *e = G + o;
-            ER B = ta0 * e[0] + ta1 * e[1];
-            if (D == 3) B = fma(ta2, e[2], B);
+            const ER *e = Gv + o;
             ER C;
             if constexpr (LY::HAS_C2) {
                 C = e[D];
@@ -143,49 +206,57 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                 C = fma(e[1], e[1], C);
                 if (D == 3) C = fma(e[2], e[2], C);
             }
-            const ER w = bond_quad<CG, NQ, ER>(R, gld, R.n, A, B, C);
-            if (VIR) {
-                const ER c = w * e[LY::OC];
-                const ER cb0 = c * e[0], cb1 = c * e[1];
-                if constexpr (std::is_same<ER, double>::value) {
-                    av[0] = fma(cb0, e[0], av[0]);
-                    av[1] = fma(cb1, e[1], av[1]);
-                    if (OD == 2) {
-                        av[2] = fma(cb0, e[1], av[2]);
+#pragma unroll
+            for (int j = 0; j < PL; ++j) {
+                if (PL > 1 && !((m >> j) & 1)) continue;
+                ER B = ta[j][0] * e[0] + ta[j][1] * e[1];
+                if (D == 3) B = fma(ta[j][2], e[2], B);
+                const ER w = bond_quad<CG, NQ, ER>(R, gld, R.n, Aj[j], B, C);
+                if (VIR) {
+                    const ER c = w * e[LY::OC];
+                    const ER cb0 = c * e[0], cb1 = c * e[1];
+                    double *v = av[j];
+                    if constexpr (std::is_same<ER, double>::value) {
+                        v[0] = fma(cb0, e[0], v[0]);
+                        v[1] = fma(cb1, e[1], v[1]);
+                        if (OD == 2) {
+                            v[2] = fma(cb0, e[1], v[2]);
+                        } else {
+                            const ER cb2 = c * e[2];
+                            v[2] = fma(cb2, e[2], v[2]);
+                            v[3] = fma(cb0, e[1], v[3]);
+                            v[4] = fma(cb0, e[2], v[4]);
+                            v[5] = fma(cb1, e[2], v[5]);
+                        }
                     } else {
-                        const ER cb2 = c * e[2];
-                        av[2] = fma(cb2, e[2], av[2]);
-                        av[3] = fma(cb0, e[1], av[3]);
-                        av[4] = fma(cb0, e[2], av[4]);
-                        av[5] = fma(cb1, e[2], av[5]);
-                    }
-                } else {
-                    av[0] += (double)(cb0 * e[0]);
-                    av[1] += (double)(cb1 * e[1]);
-                    if (OD == 2) {
-                        av[2] += (double)(cb0 * e[1]);
-                    } else {
-                        const ER cb2 = c * e[2];
-                        av[2] += (double)(cb2 * e[2]);
-                        av[3] += (double)(cb0 * e[1]);
-                        av[4] += (double)(cb0 * e[2]);
-                        av[5] += (double)(cb1 * e[2]);
+                        v[0] += (double)(cb0 * e[0]);
+                        v[1] += (double)(cb1 * e[1]);
+                        if (OD == 2) {
+                            v[2] += (double)(cb0 * e[1]);
+                        } else {
+                            const ER cb2 = c * e[2];
+                            v[2] += (double)(cb2 * e[2]);
+                            v[3] += (double)(cb0 * e[1]);
+                            v[4] += (double)(cb0 * e[2]);
+                            v[5] += (double)(cb1 * e[2]);
+                        }
                     }
                 }
-            }
-            if (ELA) {
+                if (ELA) {
 #pragma unroll
-                for (int m = 0; m < KN; ++m) {
-                    if constexpr (std::is_same<ER, double>::value) ae[m] = fma(w, e[LY::OE + m], ae[m]);
-                    else ae[m] += (double)(w * e[LY::OE + m]);
+                    for (int q = 0; q < KN; ++q) {
+                        if constexpr (std::is_same<ER, double>::value) ae[j][q] = fma(w, e[LY::OE + q], ae[j][q]);
+                        else ae[j][q] += (double)(w * e[LY::OE + q]);
+                    }
                 }
             }
         };
         // the entries of one group that exist (k-th group of a particle with nb)
-        auto integrate_group = [&](const ER (&G)[GV], int k, int nb, ER ta0, ER ta1, ER ta2, ER A) {
-            integrate(G, std::integral_constant<int, 0>(), ta0, ta1, ta2, A);
+        auto integrate_group = [&](const ER (&Gv)[GV], int k, int nb, int m, const ER (&ta)[PL][3],
+                                   const ER (&Aj)[PL]) {
+            integrate(Gv, std::integral_constant<int, 0>(), m, ta, Aj);
             if constexpr (GRP == 2)
-                if (2 * k + 1 < nb) integrate(G, std::integral_constant<int, ES>(), ta0, ta1, ta2, A);
+                if (2 * k + 1 < nb) integrate(Gv, std::integral_constant<int, ES>(), m, ta, Aj);
         };
 
         // pass B over the first cnt list entries
@@ -196,17 +267,20 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                     const int ve = lp[k][lane];
                     const int vn = getnb(k);
                     const unsigned short vs = PACK ? 0 : lnb[k][lane];
+                    const unsigned char vm = (PL > 1 && !PACK) ? lmk[k][lane] : 0;
                     int m = k - 1;
                     while (m >= 0 && getnb(m) < vn) {
                         lp[m + 1][lane] = lp[m][lane];
                         if constexpr (!PACK) lnb[m + 1][lane] = lnb[m][lane];
+                        if constexpr (PL > 1 && !PACK) lmk[m + 1][lane] = lmk[m][lane];
                         --m;
                     }
                     lp[m + 1][lane] = ve;
                     if constexpr (!PACK) lnb[m + 1][lane] = vs;
+                    if constexpr (PL > 1 && !PACK) lmk[m + 1][lane] = vm;
                 }
             }
-            // particle data and first bond entry of list entry q
+            // particle data and first bond entries of list entry q
             auto fetch = [&](int q, double (&ps)[PS], double (&kv)[KN], ER (&e0)[GV]) {
                 const int pp = getp(q);
 #pragma unroll
@@ -221,24 +295,32 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
             fetch(0, pos, kk, E0);
             for (int q = 0; q < cnt; ++q) {
                 const int p = getp(q);
+                const int m = getm(q);
                 // the next particle's data, loaded ahead
                 double npos[PS], nkk[KN];
                 ER nE0[GV];
                 fetch(q + 1 < cnt ? q + 1 : q, npos, nkk, nE0);
 
-                const double a0 = minimg(pos[0] - x0, g.L[0], g.invL[0], g.periodic[0]);
-                const double a1 = minimg(pos[1] - x1, g.L[1], g.invL[1], g.periodic[1]);
-                const double a2 = (D == 3) ? minimg(pos[2] - x2, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
-                const double A = (a0 * a0 + a1 * a1 + a2 * a2) * g.inv_rc2;
-                if (KIN) {
-                    const double w = shape<CG, double>(A);
+                ER ta[PL][3], Aj[PL];
 #pragma unroll
-                    for (int k = 0; k < KN; ++k) ak[k] = fma(w, kk[k], ak[k]);
+                for (int j = 0; j < PL; ++j) {
+                    const double a0 = minimg(pos[0] - xg[j][0], g.L[0], g.invL[0], g.periodic[0]);
+                    const double a1 = minimg(pos[1] - xg[j][1], g.L[1], g.invL[1], g.periodic[1]);
+                    const double a2 = (D == 3) ? minimg(pos[2] - xg[j][2], g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                    const double A = (a0 * a0 + a1 * a1 + a2 * a2) * g.inv_rc2;
+                    if (KIN && (PL == 1 || ((m >> j) & 1))) {
+                        const double w = shape<CG, double>(A);
+#pragma unroll
+                        for (int k = 0; k < KN; ++k) ak[j][k] = fma(w, kk[k], ak[j][k]);
+                    }
+                    ta[j][0] = (ER)(a0 * s2);
+                    ta[j][1] = (ER)(a1 * s2);
+                    ta[j][2] = (ER)(a2 * s2);
+                    Aj[j] = (ER)A;
                 }
                 if constexpr (PAIRS) {
                     const int nb = getnb(q);
-                    ncontrib += nb;
-                    const ER ta0 = (ER)(a0 * s2), ta1 = (ER)(a1 * s2), ta2 = (ER)(a2 * s2), Ae = (ER)A;
+                    ncontrib += (unsigned long long)nb * (PL == 1 ? 1 : __popc(m));
                     const ER *ptr = bd + (size_t)p * cap * ES;
                     // read groups k and k+1 alternate between E0 and E1, each
                     // loaded one step before it is used
@@ -247,11 +329,11 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                     int k = 0;
                     for (; k + 1 < ng; k += 2) {
                         load_entry<ER, GV>(ptr + (size_t)(k + 1) * GV, E1);
-                        integrate_group(E0, k, nb, ta0, ta1, ta2, Ae);
+                        integrate_group(E0, k, nb, m, ta, Aj);
                         load_entry<ER, GV>(ptr + (size_t)(k + 2 < ng ? k + 2 : k + 1) * GV, E0);
-                        integrate_group(E1, k + 1, nb, ta0, ta1, ta2, Ae);
+                        integrate_group(E1, k + 1, nb, m, ta, Aj);
                     }
-                    if (k < ng) integrate_group(E0, k, nb, ta0, ta1, ta2, Ae);
+                    if (k < ng) integrate_group(E0, k, nb, m, ta, Aj);
                 }
 #pragma unroll
                 for (int a = 0; a < PS; ++a) pos[a] = npos[a];
@@ -264,55 +346,85 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
             }
         };
 
-        // pass A: candidates in batches of NB, the batch's loads issued
-        // together (a load whose result the next step needs would hold the
-        // lane for a full memory latency per candidate)
+        // pass A: the cells around the lane's points, candidates in batches
+        // of NB with the batch's loads issued together (a load whose result
+        // the next step needs would hold the lane for a full memory latency
+        // per candidate)
         constexpr int NB = 8;
         static_assert(MAXL >= NB, "list shorter than a batch");
-        const int cx = cell_coord(g, 0, x0), cy = cell_coord(g, 1, x1), cz = cell_coord(g, 2, x2);
-        int rows[9];
-        const int nrows = neighbour_rows(g, cy, cz, rows);
+        int wlo[3], wn[3];
+        {
+            int cc[3][PL];
+#pragma unroll
+            for (int j = 0; j < PL; ++j) {
+                cc[0][j] = cell_coord(g, 0, xg[j][0]);
+                cc[1][j] = cell_coord(g, 1, xg[j][1]);
+                cc[2][j] = cell_coord(g, 2, xg[j][2]);
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) axis_window<PL>(g.nc[a], cc[a], pv, wlo[a], wn[a]);
+        }
+        const int nx = g.nc[0], ny = g.nc[1], nz = g.nc[2];
         int cnt = 0;
-        for (int r = 0; r < nrows; ++r) {
-            int rb[2], re[2];
-            const int nr = row_ranges(g, st, rows[r], cx, rb, re);
-            for (int s = 0; s < nr; ++s) {
-                for (int p0 = rb[s]; p0 < re[s]; p0 += NB) {
-                    if (cnt > MAXL - NB) {  // room for a whole batch
-                        process(cnt);
-                        cnt = 0;
-                    }
-                    double q[NB][PS];
-                    int qb[NB];
+        for (int iz = 0; iz < wn[2]; ++iz) {
+            const int z = wlo[2] + iz < nz ? wlo[2] + iz : wlo[2] + iz - nz;
+            for (int iy = 0; iy < wn[1]; ++iy) {
+                const int y = wlo[1] + iy < ny ? wlo[1] + iy : wlo[1] + iy - ny;
+                const int rowbase = (z * ny + y) * nx;
+                // cells wlo[0] .. wlo[0] + wn[0] - 1 of the row, wrapping
+                int rb[2], re[2], nr = 1;
+                const int xe = wlo[0] + wn[0];
+                rb[0] = st[rowbase + wlo[0]];
+                if (xe <= nx) {
+                    re[0] = st[rowbase + xe];
+                } else {
+                    re[0] = st[rowbase + nx];
+                    rb[1] = st[rowbase];
+                    re[1] = st[rowbase + xe - nx];
+                    nr = 2;
+                }
+                for (int s = 0; s < nr; ++s) {
+                    for (int p0 = rb[s]; p0 < re[s]; p0 += NB) {
+                        if (cnt > MAXL - NB) {  // room for a whole batch
+                            process(cnt);
+                            cnt = 0;
+                        }
+                        double q[NB][PS];
+                        int qb[NB];
 #pragma unroll
-                    for (int u = 0; u < NB; ++u) {
-                        const int p = min(p0 + u, re[s] - 1);
+                        for (int u = 0; u < NB; ++u) {
+                            const int p = min(p0 + u, re[s] - 1);
 #pragma unroll
-                        for (int a = 0; a < PS; ++a) q[u][a] = sp[(size_t)p * PS + a];
-                        if constexpr (PAIRS) qb[u] = bc[p];
-                    }
+                            for (int a = 0; a < PS; ++a) q[u][a] = sp[(size_t)p * PS + a];
+                            if constexpr (PAIRS) qb[u] = bc[p];
+                        }
 #pragma unroll
-                    for (int u = 0; u < NB; ++u) {
-                        const int p = p0 + u;
-                        if (p >= re[s]) break;
-                        const double a0 = minimg(q[u][0] - x0, g.L[0], g.invL[0], g.periodic[0]);
-                        const double a1 = minimg(q[u][1] - x1, g.L[1], g.invL[1], g.periodic[1]);
-                        const double a2 = (D == 3) ? minimg(q[u][2] - x2, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
-                        const double asq = a0 * a0 + a1 * a1 + a2 * a2;
-                        if (!(asq < g.rc2)) continue;
-                        ++nball;
-                        if constexpr (PAIRS) {
-                            const int nb = min(qb[u], cap);
+                        for (int u = 0; u < NB; ++u) {
+                            const int p = p0 + u;
+                            if (p >= re[s]) break;
+                            int m = 0;
+#pragma unroll
+                            for (int j = 0; j < PL; ++j) {
+                                const double a0 = minimg(q[u][0] - xg[j][0], g.L[0], g.invL[0], g.periodic[0]);
+                                const double a1 = minimg(q[u][1] - xg[j][1], g.L[1], g.invL[1], g.periodic[1]);
+                                const double a2 =
+                                    (D == 3) ? minimg(q[u][2] - xg[j][2], g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                                const double asq = a0 * a0 + a1 * a1 + a2 * a2;
+                                if (pv[j] && asq < g.rc2) m |= 1 << j;
+                            }
+                            if (!m) continue;
+                            nball += PL == 1 ? 1 : __popc(m);
+                            const int nb = PAIRS ? min(qb[u], cap) : 0;
                             if constexpr (PACK) {
-                                lp[cnt][lane] = p | (nb << 26);
+                                lp[cnt][lane] = (int)((unsigned)p | ((unsigned)nb << IB) |
+                                                      (PL > 1 ? ((unsigned)m << (IB + 6)) : 0u));
                             } else {
                                 lp[cnt][lane] = p;
-                                lnb[cnt][lane] = (unsigned short)nb;
+                                if constexpr (PAIRS) lnb[cnt][lane] = (unsigned short)nb;
+                                if constexpr (PL > 1) lmk[cnt][lane] = (unsigned char)m;
                             }
-                        } else {
-                            lp[cnt][lane] = p;
+                            ++cnt;
                         }
-                        ++cnt;
                     }
                 }
             }
@@ -321,8 +433,7 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
     }
 
     if (live) {
-        const size_t ob = ((size_t)f * G + gp) * OD * OD;
-        auto store = [&](double *dst, const double *acc, double scale) {
+        auto store = [&](double *dst, size_t ob, const double *acc, double scale) {
             for (int a = 0; a < OD; ++a) dst[ob + a + OD * a] = acc[a] * scale;
             if (OD == 2) {
                 dst[ob + 1] = acc[2] * scale;
@@ -333,9 +444,14 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                 dst[ob + 5] = dst[ob + 7] = acc[5] * scale;
             }
         };
-        if (VIR) store(out.vir, av, 1.0);
-        if (ELA) store(out.ela, ae, 1.0);
-        if (KIN) store(out.kin, ak, cgnorm);
+#pragma unroll
+        for (int j = 0; j < PL; ++j) {
+            if (!pv[j]) continue;
+            const size_t ob = ((size_t)f * G + gp[j]) * OD * OD;
+            if (VIR) store(out.vir, ob, av[j], 1.0);
+            if (ELA) store(out.ela, ob, ae[j], 1.0);
+            if (KIN) store(out.kin, ob, ak[j], cgnorm);
+        }
     }
     // counts: one plain store per wavefront
     nball = wave_sum(nball);

@@ -106,6 +106,30 @@ __device__ __forceinline__ int xcd_tile(int x, int nx)
     return (nx & 7) ? x : (x & 7) * (nx >> 3) + (x >> 3);
 }
 
+// Frames dealt to XCDs whole: the workgroups of frame f all run on XCD
+// f mod 8 (workgroups go to the XCDs round-robin by linear id), so one
+// frame's particle arrays, counters and partly written lines stay in one
+// XCD's L2.  Launch with gridDim.y = frame_rows(nf); returns false for the
+// padding workgroups (frames nf .. 8*ceil(nf/8)-1).
+#ifndef PGT_XCD_FRAMES
+#define PGT_XCD_FRAMES 1
+#endif
+__host__ __device__ inline int frame_rows(int nf) { return PGT_XCD_FRAMES ? (nf + 7) / 8 * 8 : nf; }
+
+__device__ __forceinline__ bool frame_block(int nf, int &f, int &bx)
+{
+    if (!PGT_XCD_FRAMES) {
+        f = blockIdx.y;
+        bx = blockIdx.x;
+        return true;
+    }
+    const int L = blockIdx.y * gridDim.x + blockIdx.x;
+    const int j = L >> 3;
+    f = (L & 7) + 8 * (j / gridDim.x);
+    bx = j % gridDim.x;
+    return f < nf;
+}
+
 // Sum over the 64 lanes of a wavefront (all lanes must be active).
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
@@ -118,11 +142,12 @@ __device__ __forceinline__ T wave_sum(T v)
 // ------------------------------------------------------------------------
 // cell list
 // ------------------------------------------------------------------------
-__global__ void k_cell_count(Geom g, int n, const double *__restrict__ pos, int *__restrict__ pcell,
+__global__ void k_cell_count(Geom g, int n, int nf, const double *__restrict__ pos, int *__restrict__ pcell,
                              int *__restrict__ pslot, int *__restrict__ ccount)
 {
-    const int f = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int f, bx;
+    if (!frame_block(nf, f, bx)) return;
+    const int i = bx * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double *p = pos + ((size_t)f * n + i) * 3;
     const int cx = cell_coord(g, 0, p[0]), cy = cell_coord(g, 1, p[1]), cz = cell_coord(g, 2, p[2]);
@@ -161,11 +186,12 @@ __global__ __launch_bounds__(1024) void k_cell_scan(int ncell, int *__restrict__
     if (t == T - 1) st[ncell] = sh[T - 1];
 }
 
-__global__ void k_cell_fill(int n, int ncell, const int *__restrict__ pcell, const int *__restrict__ pslot,
+__global__ void k_cell_fill(int n, int nf, int ncell, const int *__restrict__ pcell, const int *__restrict__ pslot,
                             const int *__restrict__ cstart, int *__restrict__ order)
 {
-    const int f = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int f, bx;
+    if (!frame_block(nf, f, bx)) return;
+    const int i = bx * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const size_t fi = (size_t)f * n + i;
     order[(size_t)f * n + cstart[(size_t)f * (ncell + 1) + pcell[fi]] + pslot[fi]] = i;
@@ -175,7 +201,7 @@ __global__ void k_cell_fill(int n, int ncell, const int *__restrict__ pcell, con
 // smaller input index), so the order inside a cell is the input order, and
 // write the cell-sorted particle arrays.
 template <int D, int OD>
-__global__ void k_cell_rank(int n, int ncell, const int *__restrict__ pcell, const int *__restrict__ cstart,
+__global__ void k_cell_rank(int n, int nf, int ncell, const int *__restrict__ pcell, const int *__restrict__ cstart,
                             const int *__restrict__ order, const double *__restrict__ pos,
                             const double *__restrict__ diam, const double *__restrict__ mass,
                             const double *__restrict__ vel, const double *__restrict__ disp,
@@ -184,8 +210,9 @@ __global__ void k_cell_rank(int n, int ncell, const int *__restrict__ pcell, con
 {
     constexpr int PS = (D == 2) ? 2 : 4;
     constexpr int KN = OD * (OD + 1) / 2;
-    const int f = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int f, bx;
+    if (!frame_block(nf, f, bx)) return;
+    const int i = bx * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const size_t fi = (size_t)f * n + i;
     const int c = pcell[fi];

@@ -20,7 +20,7 @@ TOL = 1e-10
 
 
 def run_both(sysd, grid, fields=("virial",), dim=None, order=5, cg="octic", cg_rcut=2.0, pot="poly12", params=None,
-             scaled_rcut=1.25, max_rcut=None, block=None, precision="fp64"):
+             scaled_rcut=1.25, max_rcut=None, block=None, precision="fp64", pl=None):
     dim = dim or sysd["dim"]
     params = params or oracle.poly12_params(rcut=scaled_rcut)
     if max_rcut is None:
@@ -30,6 +30,8 @@ def run_both(sysd, grid, fields=("virial",), dim=None, order=5, cg="octic", cg_r
                       max_rcut=max_rcut, precision=precision)
     if block:
         e.set_tuning(threads_per_block=block)
+    if pl:
+        e.set_tuning(points_per_lane=pl)
     e.set_system(sysd, grid)
     got = e.compute(fields=fields)
     st = e.stats()
@@ -64,6 +66,79 @@ def test_block_sizes_agree(gpu, block):
     sysd, grid = workload.config_frame(1, 1)
     got, ref, st = run_both(sysd, grid, fields=("virial", "kinetic"), block=block)
     assert_parity(got, ref, st, ("virial", "kinetic"))
+
+
+# ---------------------------------------------------------------------------
+# several grid points per gather lane (one read of each particle and bond
+# entry for all the lane's points)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("block", [128, 256])
+@pytest.mark.parametrize("pl", [1, 2, 4])
+def test_points_per_lane(gpu, pl, block):
+    sysd, grid = workload.config_frame(1, 1)
+    got, ref, st = run_both(sysd, grid, fields=("virial", "kinetic"), block=block, pl=pl)
+    assert_parity(got, ref, st, ("virial", "kinetic"))
+
+
+@pytest.mark.parametrize("pl", [2, 4])
+def test_points_per_lane_ragged_and_scattered_grids(gpu, pl):
+    """37 x 29 points (no multiple of 2 or 4: the last lane has missing
+    points) and uniformly random points (neighbours in Morton order are not
+    lattice neighbours, so a lane's cell window is wide)."""
+    sysd, _ = workload.config_frame(1, 6)
+    L = sysd["L"][0]
+    xs, ys = np.linspace(-L / 2, L / 2, 37), np.linspace(-L / 2, L / 2, 29)
+    g1 = np.zeros((37 * 29, 3))
+    g1[:, 0] = np.repeat(xs, 29)
+    g1[:, 1] = np.tile(ys, 37)
+    g2 = np.zeros((501, 3))
+    g2[:, :2] = np.random.default_rng(8).uniform(-L / 2, L / 2, (501, 2))
+    for grid in (g1, g2):
+        got, ref, st = run_both(sysd, grid, fields=("virial", "kinetic"), pl=pl)
+        assert_parity(got, ref, st, ("virial", "kinetic"))
+
+
+@pytest.mark.parametrize("pl", [2, 4])
+@pytest.mark.parametrize("n", [49, 81])
+def test_points_per_lane_few_cells(gpu, pl, n):
+    """Boxes of 3 and 4 cells per axis: the windows of a lane's points wrap
+    around the periodic box and reach a whole row."""
+    sysd = workload.make_system(n, 2, seed=13 + n)
+    grid = workload.cartesian_grid_2d(sysd["L"][0], 12)
+    got, ref, st = run_both(sysd, grid, fields=("virial", "kinetic"), pl=pl)
+    assert_parity(got, ref, st, ("virial", "kinetic"))
+
+
+@pytest.mark.parametrize("pl", [2, 4])
+def test_points_per_lane_3d_elastic_fp32(gpu, pl):
+    s3 = workload.make_system(3000, 3, seed=78)
+    grid = workload.cartesian_grid_3d(s3["L"][0], 9)
+    got, ref, st = run_both(s3, grid, fields=("virial", "kinetic"), pl=pl)
+    assert_parity(got, ref, st, ("virial", "kinetic"))
+    sysd, grid = workload.config_frame(1, 5)
+    sysd["disp"] = np.random.default_rng(4).normal(0, 0.05, (len(sysd["diam"]), 3))
+    sysd["disp"][:, 2] = 0
+    got, ref, st = run_both(sysd, grid, fields=("virial", "elastic"), pl=pl)
+    assert_parity(got, ref, st, ("virial",), count_contrib=False)
+    err = np.abs(got["elastic"] - ref["elastic"]) / max(np.abs(ref["elastic"]).max(), 1e-300)
+    assert err.max() <= 1e-12
+    sysd, grid = workload.config_frame(1, 0)
+    got, ref, st = run_both(sysd, grid, fields=("virial",), precision="fp32", pl=pl)
+    rel = np.abs(got["virial"] - ref["virial"]).max() / np.abs(ref["virial"]).max()
+    print("fp32 form, %d points per lane: max|err|/max|T| = %.3e" % (pl, rel))
+    assert rel < 1e-5
+
+
+def test_points_per_lane_ignored_off_the_unrolled_path(gpu):
+    """Rect kernel and a 2D box with nonzero z run one point per lane
+    whatever the setting."""
+    sysd, grid = workload.config_frame(1, 3)
+    got, ref, st = run_both(sysd, grid[::5], fields=("virial",), cg="rect", pl=4)
+    assert_parity(got, ref, st, ("virial",))
+    sysd = workload.make_system(1500, 2, seed=6)
+    sysd["pos"][:, 2] = np.random.default_rng(2).normal(0, 0.05, len(sysd["diam"]))
+    got, ref, st = run_both(sysd, workload.cartesian_grid_2d(sysd["L"][0], 20), fields=("virial",), pl=4)
+    assert_parity(got, ref, st, ("virial",))
 
 
 def _fixture_frame(k):
