@@ -232,7 +232,7 @@ struct pgt_ik_engine {
     DBuf<double> d_pos, d_diam, d_mass, d_vel, d_disp;  // single-frame host uploads
     DBuf<double> d_grid;
     DBuf<int> d_perm;
-    DBuf<int> d_pcell, d_pslot, d_ccount, d_cstart, d_order, d_bcount, d_maxcount;
+    DBuf<int> d_keys0, d_vals0, d_keys1, d_vals1, d_hist, d_cstart, d_bcount, d_maxcount;  // cell list, bond counts
     DBuf<double> d_spos, d_sdiam, d_skin, d_sdisp;
     DBuf<unsigned char> d_bonds;
     DBuf<double> d_out;
@@ -483,8 +483,16 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     else if (OD == 2) ebytes = bond_entry_bytes<3, 2, double>(fields);
     else ebytes = fp32 ? bond_entry_bytes<3, 3, float>(fields) : bond_entry_bytes<3, 3, double>(fields);
     int rc;
-    if ((rc = need(e->d_pcell, (size_t)nf * n)) || (rc = need(e->d_pslot, (size_t)nf * n)) ||
-        (rc = need(e->d_order, (size_t)nf * n)) || (rc = need(e->d_ccount, (size_t)nf * g.ncell)) ||
+    // radix sort of the cell keys: digits of nbits bits, npass passes
+    int kbits = 0;
+    while ((1 << kbits) < g.ncell) ++kbits;
+    const int npass = std::max(1, (kbits + 7) / 8);
+    const int nbits = std::max(1, (kbits + npass - 1) / npass);
+    const int radix = 1 << nbits;
+    const int nchunk = (n + kSortChunk - 1) / kSortChunk;
+    if ((rc = need(e->d_keys0, (size_t)nf * n)) || (rc = need(e->d_vals0, (size_t)nf * n)) ||
+        (rc = need(e->d_keys1, (size_t)nf * n)) || (rc = need(e->d_vals1, (size_t)nf * n)) ||
+        (rc = need(e->d_hist, (size_t)nf * radix * std::max(nchunk, 1))) ||
         (rc = need(e->d_cstart, (size_t)nf * (g.ncell + 1))) || (rc = need(e->d_spos, (size_t)nf * n * PS)) ||
         (rc = need(e->d_sdiam, (size_t)nf * n)) || (rc = need(e->d_bcount, (size_t)nf * n)))
         return rc;
@@ -494,23 +502,40 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     if (ev_begin) HIPCHK(hipEventRecord(ev_begin, s));
     double *skin = kin ? e->d_skin.p : nullptr;
     double *sdisp = ela ? e->d_sdisp.p : nullptr;
-    if ((rc = fill_async(e->d_ccount.p, 0, (size_t)nf * g.ncell, s))) return rc;
-    // frame-per-XCD layout of the cell-list kernels (frame_block)
-    const dim3 pgrid((unsigned)((n + 255) / 256), (unsigned)frame_rows(nf));
-    hipLaunchKernelGGL(k_cell_count, pgrid, dim3(256), 0, s, g, n, nf, pos, e->d_pcell.p, e->d_pslot.p,
-                       e->d_ccount.p);
-    hipLaunchKernelGGL(k_cell_scan, dim3(nf), dim3(1024), 0, s, g.ncell, e->d_ccount.p, e->d_cstart.p);
-    hipLaunchKernelGGL(k_cell_fill, pgrid, dim3(256), 0, s, n, nf, g.ncell, e->d_pcell.p, e->d_pslot.p,
-                       e->d_cstart.p, e->d_order.p);
-    if (D == 2)
-        hipLaunchKernelGGL((k_cell_rank<2, 2>), pgrid, dim3(256), 0, s, n, nf, g.ncell, e->d_pcell.p, e->d_cstart.p,
-                           e->d_order.p, pos, diam, mass, vel, disp, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
-    else if (OD == 2)
-        hipLaunchKernelGGL((k_cell_rank<3, 2>), pgrid, dim3(256), 0, s, n, nf, g.ncell, e->d_pcell.p, e->d_cstart.p,
-                           e->d_order.p, pos, diam, mass, vel, disp, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
-    else
-        hipLaunchKernelGGL((k_cell_rank<3, 3>), pgrid, dim3(256), 0, s, n, nf, g.ncell, e->d_pcell.p, e->d_cstart.p,
-                           e->d_order.p, pos, diam, mass, vel, disp, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
+    // cell list: per pass a histogram, a scan and a scatter of one digit of
+    // the cell keys; frames are dealt to XCDs whole (frame_block)
+    {
+        const dim3 cgrid((unsigned)nchunk, (unsigned)frame_rows(nf));
+        DBuf<int> *kb[2] = {&e->d_keys0, &e->d_keys1}, *vb[2] = {&e->d_vals0, &e->d_vals1};
+        for (int p = 0; p < npass; ++p) {
+            const int shift = p * nbits;
+            int *kin = kb[p & 1]->p, *vin = vb[p & 1]->p, *kout = kb[(p + 1) & 1]->p, *vout = vb[(p + 1) & 1]->p;
+            if (p == 0) {
+                hipLaunchKernelGGL(k_sort_hist<true>, cgrid, dim3(kSortBlock), 0, s, g, n, nf, shift, radix, pos, kin,
+                                   e->d_hist.p);
+                hipLaunchKernelGGL(k_sort_scan, dim3(nf), dim3(1024), 0, s, radix * nchunk, e->d_hist.p);
+                hipLaunchKernelGGL(k_sort_scatter<true>, cgrid, dim3(kSortBlock), 0, s, n, nf, shift, radix, kin, vin,
+                                   e->d_hist.p, kout, vout);
+            } else {
+                hipLaunchKernelGGL(k_sort_hist<false>, cgrid, dim3(kSortBlock), 0, s, g, n, nf, shift, radix, pos,
+                                   kin, e->d_hist.p);
+                hipLaunchKernelGGL(k_sort_scan, dim3(nf), dim3(1024), 0, s, radix * nchunk, e->d_hist.p);
+                hipLaunchKernelGGL(k_sort_scatter<false>, cgrid, dim3(kSortBlock), 0, s, n, nf, shift, radix, kin,
+                                   vin, e->d_hist.p, kout, vout);
+            }
+        }
+        const int *keys = kb[npass & 1]->p, *vals = vb[npass & 1]->p;
+        const dim3 pgrid((unsigned)((n + 255) / 256), (unsigned)frame_rows(nf));
+        if (D == 2)
+            hipLaunchKernelGGL((k_sort_finish<2, 2>), pgrid, dim3(256), 0, s, n, nf, g.ncell, keys, vals, pos, diam,
+                               mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
+        else if (OD == 2)
+            hipLaunchKernelGGL((k_sort_finish<3, 2>), pgrid, dim3(256), 0, s, n, nf, g.ncell, keys, vals, pos, diam,
+                               mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
+        else
+            hipLaunchKernelGGL((k_sort_finish<3, 3>), pgrid, dim3(256), 0, s, n, nf, g.ncell, keys, vals, pos, diam,
+                               mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
+    }
     HIPCHK(hipGetLastError());
 
     LaunchCtx c{};
@@ -766,8 +791,8 @@ void pgt_ik_destroy(pgt_ik_engine *e)
     DBuf<double> *dbufs[] = {&e->d_pos, &e->d_diam, &e->d_mass, &e->d_vel, &e->d_disp, &e->d_grid,
                              &e->d_spos, &e->d_sdiam, &e->d_skin, &e->d_sdisp, &e->d_out, &e->d_gl};
     for (auto *b : dbufs) b->release();
-    DBuf<int> *ibufs[] = {&e->d_perm, &e->d_pcell, &e->d_pslot, &e->d_ccount, &e->d_cstart,
-                          &e->d_order, &e->d_bcount, &e->d_maxcount};
+    DBuf<int> *ibufs[] = {&e->d_perm,  &e->d_keys0,  &e->d_vals0,  &e->d_keys1,   &e->d_vals1,
+                          &e->d_hist, &e->d_cstart, &e->d_bcount, &e->d_maxcount};
     for (auto *b : ibufs) b->release();
     e->d_bonds.release();
     e->d_stats.release();

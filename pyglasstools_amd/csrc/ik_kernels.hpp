@@ -1,7 +1,7 @@
 // ik_kernels.hpp -- the GPU kernels of the IK stress-field engine (gfx950).
 // Included once, by ik_engine.hip.
 //
-//   cell list   k_cell_count -> k_cell_scan -> k_cell_fill -> k_cell_rank
+//   cell list   k_sort_hist -> k_sort_scan -> k_sort_scatter (per digit), k_sort_finish
 //   bond list   k_bonds   (ik_bonds.hpp)
 //   fields      k_gather  (ik_gather.hpp); with the adaptive bond integral
 //               k_gather_adaptive (ik_gather_adaptive.hpp)
@@ -111,18 +111,10 @@ __device__ __forceinline__ int xcd_tile(int x, int nx)
 // frame's particle arrays, counters and partly written lines stay in one
 // XCD's L2.  Launch with gridDim.y = frame_rows(nf); returns false for the
 // padding workgroups (frames nf .. 8*ceil(nf/8)-1).
-#ifndef PGT_XCD_FRAMES
-#define PGT_XCD_FRAMES 1
-#endif
-__host__ __device__ inline int frame_rows(int nf) { return PGT_XCD_FRAMES ? (nf + 7) / 8 * 8 : nf; }
+__host__ __device__ inline int frame_rows(int nf) { return (nf + 7) / 8 * 8; }
 
 __device__ __forceinline__ bool frame_block(int nf, int &f, int &bx)
 {
-    if (!PGT_XCD_FRAMES) {
-        f = blockIdx.y;
-        bx = blockIdx.x;
-        return true;
-    }
     const int L = blockIdx.y * gridDim.x + blockIdx.x;
     const int j = L >> 3;
     f = (L & 7) + 8 * (j / gridDim.x);
@@ -140,34 +132,65 @@ __device__ __forceinline__ T wave_sum(T v)
 }
 
 // ------------------------------------------------------------------------
-// cell list
+// cell list: a stable LSD radix sort of (cell, input index) per frame.
+// Every pass keeps the input order among equal digits, so particles end up
+// in cell order and, inside a cell, in input order: the cell-sorted arrays
+// do not depend on scheduling.  No global atomics are used.
 // ------------------------------------------------------------------------
-__global__ void k_cell_count(Geom g, int n, int nf, const double *__restrict__ pos, int *__restrict__ pcell,
-                             int *__restrict__ pslot, int *__restrict__ ccount)
+constexpr int kSortBlock = 256;                     // threads per workgroup
+constexpr int kSortRows = 16;                       // rows of kSortBlock items per chunk
+constexpr int kSortChunk = kSortBlock * kSortRows;  // items per chunk (one workgroup)
+constexpr int kSortMaxRadix = 256;                  // digits of at most 8 bits
+
+__device__ __forceinline__ int cell_of(const Geom &g, const double *p)
 {
-    int f, bx;
-    if (!frame_block(nf, f, bx)) return;
-    const int i = bx * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double *p = pos + ((size_t)f * n + i) * 3;
     const int cx = cell_coord(g, 0, p[0]), cy = cell_coord(g, 1, p[1]), cz = cell_coord(g, 2, p[2]);
-    const int c = (cz * g.nc[1] + cy) * g.nc[0] + cx;
-    pcell[(size_t)f * n + i] = c;
-    pslot[(size_t)f * n + i] = atomicAdd(&ccount[(size_t)f * g.ncell + c], 1);
+    return (cz * g.nc[1] + cy) * g.nc[0] + cx;
 }
 
-// Exclusive scan of the cell counts into cstart (ncell+1 entries per frame),
-// one 1024-thread workgroup per frame; the counts are cleared for reuse.
-__global__ __launch_bounds__(1024) void k_cell_scan(int ncell, int *__restrict__ ccount, int *__restrict__ cstart)
+// Count the digit (key >> shift) & (radix - 1) over one chunk of one frame
+// into hist[f][digit][chunk].  With FIRST the keys are the particles' cells,
+// computed here from the input positions and stored in keys.
+template <bool FIRST>
+__global__ __launch_bounds__(kSortBlock) void k_sort_hist(Geom g, int n, int nf, int shift, int radix,
+                                                          const double *__restrict__ pos, int *__restrict__ keys,
+                                                          int *__restrict__ hist)
 {
-    const int f = blockIdx.x;
-    int *cnt = ccount + (size_t)f * ncell;
-    int *st = cstart + (size_t)f * (ncell + 1);
+    __shared__ int h[kSortMaxRadix];
+    int f, ch;
+    if (!frame_block(nf, f, ch)) return;
+    const int t = threadIdx.x;
+    h[t] = 0;
+    __syncthreads();
+    int *kf = keys + (size_t)f * n;
+    const int base = ch * kSortChunk;
+    for (int r = 0; r < kSortRows; ++r) {
+        const int i = base + r * kSortBlock + t;
+        if (i >= n) break;
+        int k;
+        if constexpr (FIRST) {
+            k = cell_of(g, pos + ((size_t)f * n + i) * 3);
+            kf[i] = k;
+        } else {
+            k = kf[i];
+        }
+        atomicAdd(&h[(k >> shift) & (radix - 1)], 1);
+    }
+    __syncthreads();
+    if (t < radix) hist[((size_t)f * radix + t) * gridDim.x + ch] = h[t];
+}
+
+// In-place exclusive scan of len ints per frame, one 1024-thread workgroup
+// per frame: hist[f][digit][chunk] becomes the first output slot of that
+// digit's items of that chunk.
+__global__ __launch_bounds__(1024) void k_sort_scan(int len, int *__restrict__ hist)
+{
+    int *a = hist + (size_t)blockIdx.x * len;
     const int T = blockDim.x, t = threadIdx.x;
-    const int chunk = (ncell + T - 1) / T;
-    const int b0 = min(ncell, t * chunk), b1 = min(ncell, b0 + chunk);
+    const int chunk = (len + T - 1) / T;
+    const int b0 = min(len, t * chunk), b1 = min(len, b0 + chunk);
     int s = 0;
-    for (int k = b0; k < b1; ++k) s += cnt[k];
+    for (int k = b0; k < b1; ++k) s += a[k];
     __shared__ int sh[1024];
     sh[t] = s;
     __syncthreads();
@@ -179,49 +202,103 @@ __global__ __launch_bounds__(1024) void k_cell_scan(int ncell, int *__restrict__
     }
     int run = sh[t] - s;
     for (int k = b0; k < b1; ++k) {
-        st[k] = run;
-        run += cnt[k];
-        cnt[k] = 0;
+        const int v = a[k];
+        a[k] = run;
+        run += v;
     }
-    if (t == T - 1) st[ncell] = sh[T - 1];
 }
 
-__global__ void k_cell_fill(int n, int nf, int ncell, const int *__restrict__ pcell, const int *__restrict__ pslot,
-                            const int *__restrict__ cstart, int *__restrict__ order)
+// Move the items of one chunk to their slots for this digit.  A chunk is
+// taken row by row (kSortBlock consecutive items); inside a row an item's
+// slot is the chunk's offset for its digit, plus the items of that digit in
+// earlier rows, in earlier wavefronts of the row, and in lower lanes of its
+// wavefront (found with one ballot per digit bit).  With FIRST the value is
+// the input index.
+template <bool FIRST>
+__global__ __launch_bounds__(kSortBlock) void k_sort_scatter(int n, int nf, int shift, int radix,
+                                                             const int *__restrict__ kin, const int *__restrict__ vin,
+                                                             const int *__restrict__ hist, int *__restrict__ kout,
+                                                             int *__restrict__ vout)
 {
-    int f, bx;
-    if (!frame_block(nf, f, bx)) return;
-    const int i = bx * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const size_t fi = (size_t)f * n + i;
-    order[(size_t)f * n + cstart[(size_t)f * (ncell + 1) + pcell[fi]] + pslot[fi]] = i;
+    constexpr int NW = kSortBlock / 64;
+    __shared__ int run[kSortMaxRadix];     // next slot of each digit
+    __shared__ int wc[NW][kSortMaxRadix];  // per-wavefront digit counts of a row, then their slots
+    int f, ch;
+    if (!frame_block(nf, f, ch)) return;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const size_t fo = (size_t)f * n;
+    if (t < radix) run[t] = hist[((size_t)f * radix + t) * gridDim.x + ch];
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;  // lanes under this one
+    int nbits = 0;
+    while ((1 << nbits) < radix) ++nbits;
+    const int base = ch * kSortChunk;
+    for (int r = 0; r < kSortRows; ++r) {
+        if (base + r * kSortBlock >= n) break;  // uniform over the workgroup
+        for (int k = t; k < NW * kSortMaxRadix; k += kSortBlock) wc[k / kSortMaxRadix][k % kSortMaxRadix] = 0;
+        __syncthreads();
+        const int i = base + r * kSortBlock + t;
+        const bool valid = i < n;
+        int key = 0, val = 0, d = 0;
+        if (valid) {
+            key = kin[fo + i];
+            val = FIRST ? i : vin[fo + i];
+            d = (key >> shift) & (radix - 1);
+        }
+        unsigned long long peers = __ballot(valid);
+        for (int b = 0; b < nbits; ++b) {
+            const bool bit = (d >> b) & 1;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int rank = __popcll(peers & below);
+        if (valid && rank == 0) wc[w][d] = __popcll(peers);
+        __syncthreads();
+        if (t < radix) {
+            int acc = run[t];
+            for (int q = 0; q < NW; ++q) {
+                const int c = wc[q][t];
+                wc[q][t] = acc;
+                acc += c;
+            }
+            run[t] = acc;
+        }
+        __syncthreads();
+        if (valid) {
+            const int slot = wc[w][d] + rank;
+            kout[fo + slot] = key;
+            vout[fo + slot] = val;
+        }
+        __syncthreads();
+    }
 }
 
-// Place particle i at its cell start + (number of cell members with a
-// smaller input index), so the order inside a cell is the input order, and
-// write the cell-sorted particle arrays.
+// Write the cell-sorted particle arrays (slot k takes input particle
+// vals[k]) and the cell starts: cstart[c] = first slot of cell c, and
+// cstart[ncell] = n (keys are sorted, so slot k writes the starts of the
+// cells after its predecessor's up to its own).
 template <int D, int OD>
-__global__ void k_cell_rank(int n, int nf, int ncell, const int *__restrict__ pcell, const int *__restrict__ cstart,
-                            const int *__restrict__ order, const double *__restrict__ pos,
-                            const double *__restrict__ diam, const double *__restrict__ mass,
-                            const double *__restrict__ vel, const double *__restrict__ disp,
-                            double *__restrict__ spos, double *__restrict__ sdiam, double *__restrict__ skin,
-                            double *__restrict__ sdisp)
+__global__ __launch_bounds__(256) void k_sort_finish(int n, int nf, int ncell, const int *__restrict__ keys,
+                                                     const int *__restrict__ vals, const double *__restrict__ pos,
+                                                     const double *__restrict__ diam, const double *__restrict__ mass,
+                                                     const double *__restrict__ vel, const double *__restrict__ disp,
+                                                     int *__restrict__ cstart, double *__restrict__ spos,
+                                                     double *__restrict__ sdiam, double *__restrict__ skin,
+                                                     double *__restrict__ sdisp)
 {
     constexpr int PS = (D == 2) ? 2 : 4;
     constexpr int KN = OD * (OD + 1) / 2;
     int f, bx;
     if (!frame_block(nf, f, bx)) return;
-    const int i = bx * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const size_t fi = (size_t)f * n + i;
-    const int c = pcell[fi];
-    const int *st = cstart + (size_t)f * (ncell + 1);
-    const int *ord = order + (size_t)f * n;
-    const int b = st[c], e = st[c + 1];
-    int rank = 0;
-    for (int k = b; k < e; ++k) rank += ord[k] < i;
-    const size_t fk = (size_t)f * n + b + rank;
+    const int k = bx * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const size_t fk = (size_t)f * n + k;
+    const int c = keys[fk];
+    const int cp = k == 0 ? -1 : keys[fk - 1];
+    int *st = cstart + (size_t)f * (ncell + 1);
+    for (int q = cp + 1; q <= c; ++q) st[q] = k;
+    if (k == n - 1)
+        for (int q = c + 1; q <= ncell; ++q) st[q] = n;
+    const size_t fi = (size_t)f * n + vals[fk];
     const double *p = pos + fi * 3;
     double *sp = spos + fk * PS;
     sp[0] = p[0];

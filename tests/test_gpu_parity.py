@@ -338,6 +338,39 @@ def test_batched_frames_all_fields_match_single(gpu):
         assert np.array_equal(outs["e"][f].cpu().numpy(), single["elastic"])
 
 
+def test_batched_frames_several_chunks_match_oracle(gpu):
+    """Eleven frames of 9000 particles in one launch: the cell-list sort runs
+    over several chunks per frame and two digit passes (46 x 46 cells), and
+    the frame count is not a multiple of the 8 XCDs the frames are dealt to."""
+    import torch
+
+    nfr, n = 11, 9000
+    systems = [workload.make_system(n, 2, seed=500 + f) for f in range(nfr)]
+    L = systems[0]["L"]
+    grid = workload.cartesian_grid_2d(L[0], 20)
+    d = float(max(s["diam"].max() for s in systems))
+    max_rcut = max(1.25 * d, d)
+    e = engine.Engine(dim=2, max_rcut=max_rcut)
+    e.set_box(L, [1, 1, 0])
+    e.set_grid(grid)
+    dev = torch.device("cuda:0")
+    stack = lambda key: torch.tensor(np.stack([s[key] for s in systems]), dtype=torch.float64, device=dev)
+    out_v = torch.zeros((nfr, len(grid), 2, 2), dtype=torch.float64, device=dev)
+    out_k = torch.zeros_like(out_v)
+    torch.cuda.synchronize()
+    e.set_tuning(frames_per_launch=nfr)
+    e.compute_frames(nfr, n, stack("pos"), stack("diam"), stack("mass"), stack("vel"), None,
+                     fields=("virial", "kinetic"), d_virial=out_v, d_kinetic=out_k)
+    torch.cuda.synchronize()
+    for f in (0, 5, nfr - 1):
+        ref = oracle.ik_compute(dict(systems[f], dim=2), grid, max_rcut=max_rcut, fields=("virial", "kinetic"),
+                                with_abs=True)
+        err = scaled_error(out_v[f].cpu().numpy(), ref["virial"], ref["virial_abs"])
+        kerr = np.abs(out_k[f].cpu().numpy() - ref["kinetic"]).max() / np.abs(ref["kinetic"]).max()
+        print("frame %d: virial scaled err %.3e, kinetic %.3e" % (f, err.max(), kerr), flush=True)
+        assert err.max() <= TOL and kerr <= TOL
+
+
 def test_fp32_form_error_reported(gpu):
     """FP32 forces and kernel weights with FP64 minimum image and accumulation:
     the error vs the FP64 oracle is measured, not assumed."""
