@@ -128,6 +128,15 @@ def precision_form(args):
     def step():
         eng.compute_frames(F, N, pos, diam, None, None, None, fields=("virial",), d_virial=out_v)
 
+    from pyglasstools_amd.autotune import Autotuner
+
+    if args.block and args.pl:
+        eng.set_tuning(args.block, F, args.pl)
+        tuned = {"threads_per_block": args.block, "frames_per_launch": F, "points_per_lane": args.pl}
+    else:
+        log("autotuning threads per workgroup x points per lane")
+        tuned, _ = Autotuner().tune(eng, step, ("c3", N, G, F), F, fpl=(F,), sync=torch.cuda.synchronize)
+        log("tuned: %s" % tuned)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -179,6 +188,7 @@ def precision_form(args):
                          "avg_launch_ms": round(avg_launch_s * 1e3, 4), "flops_per_launch": flops},
             "cpu_baseline": None,
             "extra": {"contributions_per_s": n_contrib / elapsed, "contributions_per_frame": n_contrib / (F * args.steps),
+                      "tuning": tuned,
                       "error_vs_fp64_oracle": {"grid_points_sampled": int(len(idx)), "frame": 0,
                                                "per_component": comps, "worst_fp32": worst}}}
     print(json.dumps(line), flush=True)

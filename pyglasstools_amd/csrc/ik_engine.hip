@@ -243,7 +243,7 @@ struct pgt_ik_engine {
     int cap = 0;
     int block = 256;
     int frames_per_launch = 8;
-    int points_per_lane = 1;  // grid points per gather lane (Octic 5-node rule; 1 elsewhere)
+    int points_per_lane = 2;  // grid points per gather lane (Octic 5-node rule; 1 elsewhere)
     pgt_ik_stats stats{};
 };
 
