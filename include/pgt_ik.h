@@ -99,6 +99,9 @@ typedef struct pgt_ik_stats {
     int32_t max_bonds;     /* largest per-particle bond count seen */
     double gather_ms;      /* HIP-event time of the field gather kernel(s), last compute */
     double total_ms;       /* HIP-event time of the whole last compute */
+    int64_t n_pairs;       /* surrogate form: (grid point, bond) pairs of the last compute */
+    int64_t n_network;     /* surrogate form: pairs whose bond integral came from the network */
+    double surrogate_ms;   /* surrogate form: HIP-event time from the pair list to the field */
 } pgt_ik_stats;
 
 typedef struct pgt_ik_engine pgt_ik_engine;
@@ -184,6 +187,45 @@ int pgt_ik_set_stream(pgt_ik_engine *e, void *hip_stream);
 int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_launch, int points_per_lane);
 
 int pgt_ik_get_stats(pgt_ik_engine *e, pgt_ik_stats *out);
+
+/* ---- learned bond-integral surrogate --------------------------------------
+ * No counterpart in the reference.  A tanh MLP 5 -> 64 -> 64 -> 64 -> 1 stands
+ * in for the Octic kernel's 5-node bond integral (FixedPointCGFunc::getBondFunc,
+ * CoarseGrainFunction.h:85-88) of a 2D system.  Input row (8 floats, the last
+ * three zero): a = (r_i - x)/r_cg, b = (r_j - x)/r_cg, r_cg; output: the
+ * unnormalised integral w = sum_k w_k (1 - t_k^2)^2_+ (multiply by the
+ * normalisation 15/(8 pi r_cg^2) for getBondFunc's value).  Parameters: FP32,
+ * pgt_mlp_nparam() of them, laid out W1[5][64] b1 W2[64][64] b2 W3[64][64] b3
+ * W4[64] b4.  All pointers are device pointers; work is enqueued on the given
+ * hipStream_t (NULL: the null stream) without host synchronisation. */
+int pgt_mlp_nparam(void);
+/* floats of workspace pgt_mlp_forward / pgt_mlp_grad need for n samples */
+size_t pgt_mlp_work_floats(int64_t n);
+/* FP32 forward: y[n] */
+int pgt_mlp_forward(const float *d_params, int64_t n, const float *d_x, float *d_y, float *d_work, void *stream);
+/* FP32 mean-squared loss against t[n] (loss[0]) and its gradient (nparam floats) */
+int pgt_mlp_grad(const float *d_params, int64_t n, const float *d_x, const float *d_t, float *d_grad, float *d_loss,
+                 float *d_work, void *stream);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8, bias-corrected; step counts from 1) */
+int pgt_mlp_adam(float *d_params, float *d_m, float *d_v, const float *d_grad, int step, float lr, void *stream);
+/* inference on the matrix cores: BF16 weights and activations, FP32 accumulation */
+int pgt_mlp_infer(const float *d_params, int64_t n, const float *d_x, float *d_y, void *stream);
+
+/* Training samples from the engine's frame and grid: every (grid point, bond)
+ * pair of IrvingKirkwood::compute's loop (IrvingKirkwood.h:80-107) with a
+ * nonzero force, as input rows (d_x, 8 floats each) and its 5-node integral
+ * (d_t).  *n_pairs gets the count; with d_x NULL only counts; PGT_ERR_RANGE if
+ * max_pairs rows are too few.  Synchronises the engine's stream. */
+int pgt_ik_bond_samples(pgt_ik_engine *e, int64_t max_pairs, float *d_x, float *d_t, int64_t *n_pairs);
+
+/* Virial field of the engine's frame (host output, G x 2 x 2) with the bond
+ * integral from the network where the error bound of the pair's class (both
+ * ends inside the ball: bound_inside; the segment leaves it: bound_crossing)
+ * is at most tol, and from the 5-node rule elsewhere (tol < 0: everywhere).
+ * Replaces IrvingKirkwood::compute (IrvingKirkwood.h:74-110) for the virial
+ * field in this form; counts in pgt_ik_get_stats (n_pairs, n_network). */
+int pgt_ik_compute_surrogate(pgt_ik_engine *e, const float *d_params, double bound_inside, double bound_crossing,
+                             double tol, double *virial);
 
 /* Batched evaluators on the GPU, host arrays in and out (n x 3 vectors).
  * pgt_eval_pair_force replaces PairPotential::getPairForceDivR / getPairEnergy /

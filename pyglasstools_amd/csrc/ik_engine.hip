@@ -239,6 +239,13 @@ struct pgt_ik_engine {
     DBuf<double> d_gl;
     DBuf<unsigned long long> d_stats, d_wcounts;  // job totals; per-wavefront counts of a batch
     DBuf<int> d_wmax;                             // per-wavefront largest bond count of a batch
+    // surrogate field form: pair counts and offsets per grid point, pair
+    // records, network inputs and outputs, final bond integrals, counter
+    DBuf<int> d_pcount, d_poff;
+    DBuf<PairRec> d_prec;
+    DBuf<float> d_pxin, d_pymlp;
+    DBuf<double> d_pw;
+    DBuf<unsigned long long> d_pn;
     std::vector<double> gl_s, gl_w;
     int cap = 0;
     int block = 256;
@@ -455,7 +462,7 @@ size_t bond_entry_bytes(unsigned fields)
 int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos, const double *diam,
                const double *mass, const double *vel, const double *disp, unsigned fields, double *ovir,
                double *okin, double *oela, bool z_flat, hipEvent_t ev_begin, hipEvent_t ev_gather, hipEvent_t ev_end,
-               bool no_alloc)
+               bool no_alloc, bool bonds_only)
 {
     // no_alloc: buffers must already be large enough (graph capture)
     auto need = [&](auto &buf, size_t count) -> int {
@@ -597,7 +604,9 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     c.pack = n < (1 << (c.pl > 1 ? 22 : 26)) && c.cap <= 63;
     if (ev_gather) HIPCHK(hipEventRecord(ev_gather, s));
     const int cg = e->prm.cg_kind;
-    if (adaptive) {
+    if (bonds_only) {
+        // cell list and bond list only (the surrogate field form reads them)
+    } else if (adaptive) {
         const AdaptQ Q{e->prm.adapt_limit, e->prm.adapt_epsabs, e->prm.adapt_epsrel};
         const dim3 ag((unsigned)((c.G + 127) / 128), (unsigned)c.nf);
         const bool V = fields & PGT_FIELD_VIRIAL, E = fields & PGT_FIELD_ELASTIC;
@@ -626,11 +635,12 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
         else launch_gather_c<3, 3, double>(c, fields, cg);
     }
     HIPCHK(hipGetLastError());
-    if (!adaptive && c.pl_launched != c.pl)
+    if (!bonds_only && !adaptive && c.pl_launched != c.pl)
         return fail(PGT_ERR_STATE, "gather launched with an unexpected number of points per lane");
     if (ev_end) HIPCHK(hipEventRecord(ev_end, s));
-    const int nrb = std::max(1, std::min(256, (std::max(nw_gather, nw_bonds) + 4095) / 4096));
-    hipLaunchKernelGGL(k_reduce_stats, dim3(nrb), dim3(1024), 0, s, nw_gather, e->d_wcounts.p, nw_bonds, e->d_wmax.p,
+    const int nwg = bonds_only ? 0 : nw_gather;
+    const int nrb = std::max(1, std::min(256, (std::max(nwg, nw_bonds) + 4095) / 4096));
+    hipLaunchKernelGGL(k_reduce_stats, dim3(nrb), dim3(1024), 0, s, nwg, e->d_wcounts.p, nw_bonds, e->d_wmax.p,
                        e->d_stats.p, e->d_maxcount.p);
     HIPCHK(hipGetLastError());
     return 0;
@@ -642,6 +652,7 @@ struct FrameSet {
     const double *pos, *diam, *mass, *vel, *disp;
     double *vir, *kin, *ela;  // device outputs, nframes * G * D * D each
     bool z_flat;
+    bool bonds_only;  // build the cell and bond lists only (no field gather)
 };
 
 // Run every batch of a job on the engine's stream, then check the bond-list
@@ -668,7 +679,7 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
                             fs.vel ? fs.vel + po * 3 : nullptr, fs.disp ? fs.disp + po * 3 : nullptr, fields,
                             fs.vir ? fs.vir + (size_t)f0 * fsz : nullptr, fs.kin ? fs.kin + (size_t)f0 * fsz : nullptr,
                             fs.ela ? fs.ela + (size_t)f0 * fsz : nullptr, fs.z_flat, e->ev[3 * b],
-                            e->ev[3 * b + 1], e->ev[3 * b + 2], false);
+                            e->ev[3 * b + 1], e->ev[3 * b + 2], false, fs.bonds_only);
             if (rc) return rc;
         }
         int flags[2] = {0, 0};
@@ -725,7 +736,7 @@ int enqueue_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
                                   fs.vir ? fs.vir + (size_t)f0 * fsz : nullptr,
                                   fs.kin ? fs.kin + (size_t)f0 * fsz : nullptr,
                                   fs.ela ? fs.ela + (size_t)f0 * fsz : nullptr, fs.z_flat, nullptr, nullptr, nullptr,
-                                  true);
+                                  true, false);
         if (rc) return rc;
     }
     return 0;
@@ -736,6 +747,83 @@ int enqueue_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
 // ==========================================================================
 // C ABI
 // ==========================================================================
+namespace {
+
+GLRule engine_rule(const pgt_ik_engine *e)
+{
+    GLRule R{};
+    R.n = (int)e->gl_s.size();
+    for (int k = 0; k < R.n && k < kMaxQArg; ++k) {
+        R.s[k] = e->gl_s[k];
+        R.w[k] = e->gl_w[k];
+    }
+    return R;
+}
+
+// The surrogate stands in for the Octic 5-node rule of a 2D (z = 0) system
+// with FP64 bond entries.
+int surrogate_check(pgt_ik_engine *e)
+{
+    if (int rc = check_engine(e)) return rc;
+    if (!e->box_set || !e->have_particles || !e->have_grid)
+        return fail(PGT_ERR_STATE, "set_box, set_particles and set_grid must precede the surrogate form");
+    if (e->prm.dim != 2 || !(e->z_flat && e->grid_z_flat))
+        return fail(PGT_ERR_UNSUPPORTED, "the bond-integral surrogate is built for 2D systems (z = 0)");
+    if (e->prm.cg_kind != PGT_CG_OCTIC || e->prm.quad_mode != PGT_QUAD_FIXED || e->gl_s.size() != 5 ||
+        e->prm.precision != PGT_PREC_FP64)
+        return fail(PGT_ERR_UNSUPPORTED,
+                    "the bond-integral surrogate stands in for the Octic kernel's 5-node rule with FP64 bond entries");
+    return 0;
+}
+
+// Cell and bond lists of the engine's frame, then the (grid point, bond)
+// pairs: counts, offsets and *np.  With write, the pair records and network
+// input rows (into xin, or the engine's buffer when xin is null).
+int prepare_pairs(pgt_ik_engine *e, int64_t *np, bool write, float *xin)
+{
+    *np = 0;
+    if (e->n == 0 || e->G == 0) return 0;
+    FrameSet fs{};
+    fs.nframes = 1;
+    fs.n = e->n;
+    fs.pos = e->d_pos.p;
+    fs.diam = e->d_diam.p;
+    fs.z_flat = true;
+    fs.bonds_only = true;
+    int rc;
+    if ((rc = run_job(e, fs, PGT_FIELD_VIRIAL))) return rc;
+    const int G = (int)e->G;
+    const Geom g = make_geom(e, 2);
+    hipStream_t s = e->stream;
+    if ((rc = e->d_pcount.ensure(G)) || (rc = e->d_poff.ensure(G))) return rc;
+    const dim3 gg((unsigned)((G + 127) / 128));
+    hipLaunchKernelGGL(k_pair_count, gg, dim3(128), 0, s, g, G, e->d_grid.p, e->d_cstart.p, e->d_spos.p, e->cap,
+                       e->d_bcount.p, e->d_pcount.p);
+    HIPCHK(hipMemcpyAsync(e->d_poff.p, e->d_pcount.p, (size_t)G * sizeof(int), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, G, e->d_poff.p);
+    int last[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&last[0], e->d_poff.p + G - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&last[1], e->d_pcount.p + G - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *np = (int64_t)last[0] + last[1];
+    if (!write || *np == 0) return 0;
+    if ((rc = e->d_prec.ensure((size_t)*np)) || (rc = e->d_pw.ensure((size_t)*np))) return rc;
+    if (!xin) {
+        if ((rc = e->d_pxin.ensure((size_t)*np * kMlpInPad))) return rc;
+        xin = e->d_pxin.p;
+    }
+    hipLaunchKernelGGL(k_pair_write, gg, dim3(128), 0, s, g, G, e->d_grid.p, e->d_cstart.p, e->d_spos.p, e->cap,
+                       e->d_bcount.p, (const double *)e->d_bonds.p, e->d_poff.p, e->prm.cg_rcut, e->d_prec.p, xin);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+size_t mlp_work_floats(int64_t n) { return (size_t)n * 322 + 2 * (size_t)((n + 255) / 256) + 2; }
+
+unsigned blocks_for(int64_t threads) { return (unsigned)((threads + 255) / 256); }
+
+}  // namespace
+
 extern "C" {
 
 const char *pgt_last_error(void) { return g_err.c_str(); }
@@ -795,6 +883,13 @@ void pgt_ik_destroy(pgt_ik_engine *e)
                           &e->d_hist, &e->d_cstart, &e->d_bcount, &e->d_maxcount};
     for (auto *b : ibufs) b->release();
     e->d_bonds.release();
+    e->d_pcount.release();
+    e->d_poff.release();
+    e->d_prec.release();
+    e->d_pxin.release();
+    e->d_pymlp.release();
+    e->d_pw.release();
+    e->d_pn.release();
     e->d_stats.release();
     e->d_wcounts.release();
     e->d_wmax.release();
@@ -1157,6 +1252,165 @@ int pgt_eval_cgfunc(int cg_kind, double cg_rcut, int quad_mode, int quad_order, 
     if (capped)
         return fail(PGT_ERR_RANGE, "an adaptive bond integral needed more than " + std::to_string(kQagMax) +
                                        " intervals (the GPU rule's workspace)");
+    return PGT_OK;
+}
+
+// ------------------------------------------------------------------------
+// learned bond-integral surrogate
+// ------------------------------------------------------------------------
+int pgt_mlp_nparam(void) { return kMlpNParam; }
+
+size_t pgt_mlp_work_floats(int64_t n) { return n > 0 ? mlp_work_floats(n) : 0; }
+
+int pgt_mlp_forward(const float *d_params, int64_t n, const float *d_x, float *d_y, float *d_work, void *stream)
+{
+    if (n < 0 || (n > 0 && (!d_params || !d_x || !d_y || !d_work))) return fail(PGT_ERR_ARG, "bad network arguments");
+    if (n == 0) return PGT_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const float *P = d_params;
+    float *H1 = d_work, *H2 = d_work + 64 * n, *H3 = d_work + 128 * n;
+    hipLaunchKernelGGL(k_dense_fwd, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpIn, kMlpW, d_x, kMlpInPad,
+                       P + kOffW1, P + kOffB1, true, H1);
+    hipLaunchKernelGGL(k_dense_fwd, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpW, kMlpW, H1, kMlpW,
+                       P + kOffW2, P + kOffB2, true, H2);
+    hipLaunchKernelGGL(k_dense_fwd, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpW, kMlpW, H2, kMlpW,
+                       P + kOffW3, P + kOffB3, true, H3);
+    hipLaunchKernelGGL(k_dense_fwd, dim3(blocks_for(n)), dim3(256), 0, s, n, kMlpW, 1, H3, kMlpW, P + kOffW4,
+                       P + kOffB4, false, d_y);
+    HIPCHK(hipGetLastError());
+    return PGT_OK;
+}
+
+int pgt_mlp_grad(const float *d_params, int64_t n, const float *d_x, const float *d_t, float *d_grad, float *d_loss,
+                 float *d_work, void *stream)
+{
+    if (n <= 0 || !d_params || !d_x || !d_t || !d_grad || !d_loss || !d_work)
+        return fail(PGT_ERR_ARG, "bad network arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const float *P = d_params;
+    float *H1 = d_work, *H2 = d_work + 64 * n, *H3 = d_work + 128 * n, *dZa = d_work + 192 * n,
+          *dZb = d_work + 256 * n, *Y = d_work + 320 * n, *dY = d_work + 321 * n;
+    double *part = reinterpret_cast<double *>(d_work + 322 * n);
+    int rc;
+    if ((rc = pgt_mlp_forward(d_params, n, d_x, Y, d_work, stream))) return rc;
+    const unsigned nb = blocks_for(n);
+    hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, s, n, Y, d_t, dY, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, (int)nb, part, n, d_loss);
+    float *g = d_grad;
+    hipLaunchKernelGGL(k_dense_bwd_weight, dim3(blocks_for(kMlpW + 1)), dim3(256), 0, s, n, kMlpW, 1, H3, kMlpW, dY,
+                       g + kOffW4, g + kOffB4);
+    hipLaunchKernelGGL(k_dense_bwd_data, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpW, 1, dY, P + kOffW4, H3,
+                       dZa);
+    hipLaunchKernelGGL(k_dense_bwd_weight, dim3(blocks_for(kMlpW * kMlpW + kMlpW)), dim3(256), 0, s, n, kMlpW, kMlpW,
+                       H2, kMlpW, dZa, g + kOffW3, g + kOffB3);
+    hipLaunchKernelGGL(k_dense_bwd_data, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpW, kMlpW, dZa, P + kOffW3,
+                       H2, dZb);
+    hipLaunchKernelGGL(k_dense_bwd_weight, dim3(blocks_for(kMlpW * kMlpW + kMlpW)), dim3(256), 0, s, n, kMlpW, kMlpW,
+                       H1, kMlpW, dZb, g + kOffW2, g + kOffB2);
+    hipLaunchKernelGGL(k_dense_bwd_data, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpW, kMlpW, dZb, P + kOffW2,
+                       H1, dZa);
+    hipLaunchKernelGGL(k_dense_bwd_weight, dim3(blocks_for(kMlpIn * kMlpW + kMlpW)), dim3(256), 0, s, n, kMlpIn,
+                       kMlpW, d_x, kMlpInPad, dZa, g + kOffW1, g + kOffB1);
+    HIPCHK(hipGetLastError());
+    return PGT_OK;
+}
+
+int pgt_mlp_adam(float *d_params, float *d_m, float *d_v, const float *d_grad, int step, float lr, void *stream)
+{
+    if (!d_params || !d_m || !d_v || !d_grad || step < 1 || !(lr > 0.0f))
+        return fail(PGT_ERR_ARG, "bad optimiser arguments");
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(kMlpNParam)), dim3(256), 0, (hipStream_t)stream, kMlpNParam, d_params,
+                       d_m, d_v, d_grad, lr, b1, b2, 1e-8f, c1, c2);
+    HIPCHK(hipGetLastError());
+    return PGT_OK;
+}
+
+int pgt_mlp_infer(const float *d_params, int64_t n, const float *d_x, float *d_y, void *stream)
+{
+    if (n < 0 || (n > 0 && (!d_params || !d_x || !d_y))) return fail(PGT_ERR_ARG, "bad network arguments");
+    if (n == 0) return PGT_OK;
+    const int64_t groups = (n + 15) / 16;  // 16 samples per wavefront step
+    const unsigned nblk = (unsigned)std::min<int64_t>((groups + 3) / 4, 2048);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(nblk), dim3(256), 0, (hipStream_t)stream, n, d_params, d_x, d_y);
+    HIPCHK(hipGetLastError());
+    return PGT_OK;
+}
+
+int pgt_ik_bond_samples(pgt_ik_engine *e, int64_t max_pairs, float *d_x, float *d_t, int64_t *n_pairs)
+{
+    if (int rc = surrogate_check(e)) return rc;
+    if (!n_pairs || max_pairs < 0 || (d_x && !d_t)) return fail(PGT_ERR_ARG, "bad sample arguments");
+    int rc;
+    int64_t np = 0;
+    if ((rc = prepare_pairs(e, &np, false, nullptr))) return rc;
+    *n_pairs = np;
+    if (!d_x || np == 0) return PGT_OK;
+    if (np > max_pairs) return fail(PGT_ERR_RANGE, "the sample buffers hold fewer rows than the frame has pairs");
+    if ((rc = prepare_pairs(e, &np, true, d_x))) return rc;
+    hipLaunchKernelGGL(k_pair_finalize, dim3(blocks_for(np)), dim3(256), 0, e->stream, np, e->d_prec.p,
+                       (const double *)e->d_bonds.p, engine_rule(e), (const float *)nullptr, 0.0, 0.0, -1.0,
+                       e->d_pw.p, d_t, (unsigned long long *)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return PGT_OK;
+}
+
+int pgt_ik_compute_surrogate(pgt_ik_engine *e, const float *d_params, double bound_inside, double bound_crossing,
+                             double tol, double *virial)
+{
+    if (int rc = surrogate_check(e)) return rc;
+    if (!d_params || !virial) return fail(PGT_ERR_ARG, "null network parameters or output");
+    if (!(bound_inside >= 0.0) || !(bound_crossing >= 0.0)) return fail(PGT_ERR_ARG, "error bounds must be >= 0");
+    const int G = (int)e->G;
+    e->stats = pgt_ik_stats{};
+    if (G == 0) return PGT_OK;
+    if (e->n == 0) {
+        std::memset(virial, 0, (size_t)G * 4 * sizeof(double));
+        return PGT_OK;
+    }
+    int rc;
+    // events 0..2 belong to the bond-list job inside prepare_pairs
+    if ((rc = ensure_events(e, 6))) return rc;
+    if ((rc = e->d_out.ensure((size_t)G * 4)) || (rc = e->d_pn.ensure(1))) return rc;
+    hipStream_t s = e->stream;
+    HIPCHK(hipEventRecord(e->ev[3], s));
+    int64_t np = 0;
+    if ((rc = prepare_pairs(e, &np, true, nullptr))) return rc;
+    const int mx = e->stats.max_bonds, cap = e->stats.bond_capacity;
+    HIPCHK(hipEventRecord(e->ev[4], s));
+    if ((rc = fill_async(e->d_pn.p, 0ull, 1, s))) return rc;
+    const bool any = tol >= 0.0 && std::min(bound_inside, bound_crossing) <= tol;
+    if (np > 0) {
+        if (any) {
+            if ((rc = e->d_pymlp.ensure((size_t)np))) return rc;
+            if ((rc = pgt_mlp_infer(d_params, np, e->d_pxin.p, e->d_pymlp.p, s))) return rc;
+        }
+        hipLaunchKernelGGL(k_pair_finalize, dim3(blocks_for(np)), dim3(256), 0, s, np, e->d_prec.p,
+                           (const double *)e->d_bonds.p, engine_rule(e), any ? (const float *)e->d_pymlp.p : nullptr,
+                           bound_inside, bound_crossing, any ? tol : -1.0, e->d_pw.p, (float *)nullptr, e->d_pn.p);
+    }
+    hipLaunchKernelGGL(k_pair_reduce, dim3((unsigned)((G + 127) / 128)), dim3(128), 0, s, G, e->d_pcount.p,
+                       e->d_poff.p, e->d_prec.p, (const double *)e->d_bonds.p, e->d_pw.p, e->d_out.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e->ev[5], s));
+    unsigned long long used = 0;
+    HIPCHK(hipMemcpyAsync(&used, e->d_pn.p, sizeof(used), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(virial, e->d_out.p, (size_t)G * 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float tot = 0.0f, pairs_ms = 0.0f;
+    (void)hipEventElapsedTime(&tot, e->ev[3], e->ev[5]);
+    (void)hipEventElapsedTime(&pairs_ms, e->ev[4], e->ev[5]);
+    e->stats.n_particles = e->n;
+    e->stats.n_grid = G;
+    e->stats.n_contrib = np;
+    e->stats.bond_capacity = cap;
+    e->stats.max_bonds = mx;
+    e->stats.total_ms = tot;
+    e->stats.n_pairs = np;
+    e->stats.n_network = (int64_t)used;
+    e->stats.surrogate_ms = pairs_ms;
     return PGT_OK;
 }
 

@@ -375,3 +375,4 @@ __global__ __launch_bounds__(1024) void k_reduce_stats(int nw, const unsigned lo
 #include "ik_bonds.hpp"
 #include "ik_gather.hpp"
 #include "ik_gather_adaptive.hpp"
+#include "ik_surrogate.hpp"

@@ -24,6 +24,8 @@ EXPORTED = [
     "pgt_ik_set_particles", "pgt_ik_set_grid", "pgt_ik_compute", "pgt_ik_compute_frames", "pgt_ik_enqueue_frames",
     "pgt_ik_finish", "pgt_ik_set_stream",
     "pgt_ik_set_tuning", "pgt_ik_get_stats", "pgt_eval_pair_force", "pgt_eval_cgfunc",
+    "pgt_mlp_nparam", "pgt_mlp_work_floats", "pgt_mlp_forward", "pgt_mlp_grad", "pgt_mlp_adam", "pgt_mlp_infer",
+    "pgt_ik_bond_samples", "pgt_ik_compute_surrogate",
 ]
 
 
@@ -56,6 +58,9 @@ class Stats(ctypes.Structure):
         ("max_bonds", ctypes.c_int32),
         ("gather_ms", ctypes.c_double),
         ("total_ms", ctypes.c_double),
+        ("n_pairs", ctypes.c_int64),
+        ("n_network", ctypes.c_int64),
+        ("surrogate_ms", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -89,9 +94,18 @@ def lib():
         L.pgt_ik_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.pgt_eval_pair_force.argtypes = [i, vp, i, d, i64, vp, vp, vp, vp, vp, vp]
         L.pgt_eval_cgfunc.argtypes = [i, d, i, i, i, d, d, i, i64, vp, vp, vp, vp, vp]
+        L.pgt_mlp_nparam.argtypes = []
+        L.pgt_mlp_work_floats.argtypes = [i64]
+        L.pgt_mlp_forward.argtypes = [vp, i64, vp, vp, vp, vp]
+        L.pgt_mlp_grad.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
+        L.pgt_mlp_adam.argtypes = [vp, vp, vp, vp, i, ctypes.c_float, vp]
+        L.pgt_mlp_infer.argtypes = [vp, i64, vp, vp, vp]
+        L.pgt_ik_bond_samples.argtypes = [vp, i64, vp, vp, ctypes.POINTER(i64)]
+        L.pgt_ik_compute_surrogate.argtypes = [vp, vp, d, d, d, vp]
         for name in EXPORTED:
             if name not in ("pgt_last_error", "pgt_device_count", "pgt_ik_destroy"):
                 getattr(L, name).restype = i
+        L.pgt_mlp_work_floats.restype = ctypes.c_size_t
         _LIB = L
     return _LIB
 
@@ -207,6 +221,23 @@ class Engine(object):
         self.set_particles(system["pos"], system["diam"], system.get("mass"), system.get("vel"), system.get("disp"))
         self.set_grid(grid)
 
+    def bond_samples(self, d_x=None, d_t=None, max_pairs=0):
+        """Training samples of the current frame and grid (device float32
+        tensors: d_x rows of 8, d_t the 5-node integrals); returns the pair
+        count (only counts when d_x is None)."""
+        n = ctypes.c_int64(0)
+        _check(lib().pgt_ik_bond_samples(self.h, max_pairs, _ptr(d_x), _ptr(d_t), ctypes.byref(n)),
+               "pgt_ik_bond_samples")
+        return n.value
+
+    def compute_surrogate(self, d_params, bound_inside, bound_crossing, tol):
+        """Virial field (G, 2, 2) with the network's bond integral where the
+        pair class's error bound is within tol, the 5-node rule elsewhere."""
+        out = np.zeros((self.G, 2, 2))
+        _check(lib().pgt_ik_compute_surrogate(self.h, _ptr(d_params), bound_inside, bound_crossing, tol, _ptr(out)),
+               "pgt_ik_compute_surrogate")
+        return out
+
     def compute(self, fields=("virial",)):
         D = self.dim
         out = {}
@@ -284,3 +315,39 @@ def eval_cgfunc(mode, x, ri, rij=None, s=None, cg="octic", rcut=2.0, order=5, ad
     _check(lib().pgt_eval_cgfunc(CG[cg], rcut, qm, order, lim, er, ea, modes[mode], n, _ptr(s), _ptr(x), _ptr(ri),
                                  _ptr(rij), _ptr(out)), "pgt_eval_cgfunc")
     return out
+
+
+# ---------------------------------------------------------------------------
+# bond-integral surrogate network (device tensors; work on torch's current
+# stream so it orders with the caller's own device work)
+# ---------------------------------------------------------------------------
+def _stream():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def mlp_nparam():
+    return lib().pgt_mlp_nparam()
+
+
+def mlp_work_floats(n):
+    return int(lib().pgt_mlp_work_floats(n))
+
+
+def mlp_forward(params, x, y, work):
+    _check(lib().pgt_mlp_forward(_ptr(params), x.shape[0], _ptr(x), _ptr(y), _ptr(work), _stream()),
+           "pgt_mlp_forward")
+
+
+def mlp_grad(params, x, t, grad, loss, work):
+    _check(lib().pgt_mlp_grad(_ptr(params), x.shape[0], _ptr(x), _ptr(t), _ptr(grad), _ptr(loss), _ptr(work),
+                              _stream()), "pgt_mlp_grad")
+
+
+def mlp_adam(params, m, v, grad, step, lr):
+    _check(lib().pgt_mlp_adam(_ptr(params), _ptr(m), _ptr(v), _ptr(grad), step, lr, _stream()), "pgt_mlp_adam")
+
+
+def mlp_infer(params, x, y):
+    _check(lib().pgt_mlp_infer(_ptr(params), x.shape[0], _ptr(x), _ptr(y), _stream()), "pgt_mlp_infer")
