@@ -818,7 +818,10 @@ int prepare_pairs(pgt_ik_engine *e, int64_t *np, bool write, float *xin)
     return 0;
 }
 
-size_t mlp_work_floats(int64_t n) { return (size_t)n * 322 + 2 * (size_t)((n + 255) / 256) + 2; }
+// H1 H2 H3 dZa dZb (64 n each), Y, dY (n each), loss partials (doubles),
+// gradient partials (kGradSplit rows of 64 * 64 + 64)
+size_t mlp_part_offset(int64_t n) { return (size_t)n * 322 + 2 * (size_t)((n + 255) / 256) + 2; }
+size_t mlp_work_floats(int64_t n) { return mlp_part_offset(n) + (size_t)kGradSplit * (kMlpW * kMlpW + kMlpW); }
 
 unsigned blocks_for(int64_t threads) { return (unsigned)((threads + 255) / 256); }
 
@@ -1297,20 +1300,24 @@ int pgt_mlp_grad(const float *d_params, int64_t n, const float *d_x, const float
     hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, s, n, Y, d_t, dY, part);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, (int)nb, part, n, d_loss);
     float *g = d_grad;
-    hipLaunchKernelGGL(k_dense_bwd_weight, dim3(blocks_for(kMlpW + 1)), dim3(256), 0, s, n, kMlpW, 1, H3, kMlpW, dY,
-                       g + kOffW4, g + kOffB4);
+    float *gpart = d_work + mlp_part_offset(n);
+    // weight and bias gradient of a layer with K inputs X (row stride ldx), N outputs, error dZ
+    auto wgrad = [&](int K, int N, const float *X, int ldx, const float *dZ, float *gW, float *gb) {
+        const int M = K * N + N;
+        hipLaunchKernelGGL(k_dense_bwd_weight, dim3(blocks_for(M), kGradSplit), dim3(256), 0, s, n, K, N, X, ldx, dZ,
+                           gpart);
+        hipLaunchKernelGGL(k_sum_split, dim3(blocks_for(M)), dim3(256), 0, s, kGradSplit, K, N, gpart, gW, gb);
+    };
+    wgrad(kMlpW, 1, H3, kMlpW, dY, g + kOffW4, g + kOffB4);
     hipLaunchKernelGGL(k_dense_bwd_data, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpW, 1, dY, P + kOffW4, H3,
                        dZa);
-    hipLaunchKernelGGL(k_dense_bwd_weight, dim3(blocks_for(kMlpW * kMlpW + kMlpW)), dim3(256), 0, s, n, kMlpW, kMlpW,
-                       H2, kMlpW, dZa, g + kOffW3, g + kOffB3);
+    wgrad(kMlpW, kMlpW, H2, kMlpW, dZa, g + kOffW3, g + kOffB3);
     hipLaunchKernelGGL(k_dense_bwd_data, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpW, kMlpW, dZa, P + kOffW3,
                        H2, dZb);
-    hipLaunchKernelGGL(k_dense_bwd_weight, dim3(blocks_for(kMlpW * kMlpW + kMlpW)), dim3(256), 0, s, n, kMlpW, kMlpW,
-                       H1, kMlpW, dZb, g + kOffW2, g + kOffB2);
+    wgrad(kMlpW, kMlpW, H1, kMlpW, dZb, g + kOffW2, g + kOffB2);
     hipLaunchKernelGGL(k_dense_bwd_data, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpW, kMlpW, dZb, P + kOffW2,
                        H1, dZa);
-    hipLaunchKernelGGL(k_dense_bwd_weight, dim3(blocks_for(kMlpIn * kMlpW + kMlpW)), dim3(256), 0, s, n, kMlpIn,
-                       kMlpW, d_x, kMlpInPad, dZa, g + kOffW1, g + kOffB1);
+    wgrad(kMlpIn, kMlpW, d_x, kMlpInPad, dZa, g + kOffW1, g + kOffB1);
     HIPCHK(hipGetLastError());
     return PGT_OK;
 }

@@ -100,23 +100,40 @@ __global__ void k_dense_bwd_data(int64_t n, int K, int N, const float *__restric
     dZp[idx] = acc * (1.0f - h * h);
 }
 
-// gW[k][o] = sum_s X[s][k] dZ[s][o] (threads k*N + o) and gb[o] = sum_s dZ[s][o]
-// (threads K*N + o), each summed over the samples in order.
+// Weight and bias gradients in two deterministic stages.  Stage 1: workgroup
+// row c (blockIdx.y, of gridDim.y) sums samples [n c / S, n (c+1) / S) into
+// part[c][k*N + o] = sum X[s][k] dZ[s][o] and part[c][K*N + o] = sum dZ[s][o].
+// Stage 2 (k_sum_split) adds the S rows in order.
+constexpr int kGradSplit = 64;
 __global__ void k_dense_bwd_weight(int64_t n, int K, int N, const float *__restrict__ X, int ldx,
-                                   const float *__restrict__ dZ, float *__restrict__ gW, float *__restrict__ gb)
+                                   const float *__restrict__ dZ, float *__restrict__ part)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int M = K * N + N;
+    if (t >= M) return;
+    const int c = blockIdx.y, S = gridDim.y;
+    const int64_t s0 = n * c / S, s1 = n * (c + 1) / S;
+    float acc = 0.0f;
     if (t < K * N) {
         const int k = t / N, o = t % N;
-        float acc = 0.0f;
-        for (int64_t s = 0; s < n; ++s) acc = fmaf(X[s * ldx + k], dZ[s * N + o], acc);
-        gW[t] = acc;
-    } else if (t < K * N + N) {
+        for (int64_t s = s0; s < s1; ++s) acc = fmaf(X[s * ldx + k], dZ[s * N + o], acc);
+    } else {
         const int o = t - K * N;
-        float acc = 0.0f;
-        for (int64_t s = 0; s < n; ++s) acc += dZ[s * N + o];
-        gb[o] = acc;
+        for (int64_t s = s0; s < s1; ++s) acc += dZ[s * N + o];
     }
+    part[(size_t)c * M + t] = acc;
+}
+
+__global__ void k_sum_split(int S, int K, int N, const float *__restrict__ part, float *__restrict__ gW,
+                            float *__restrict__ gb)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int M = K * N + N;
+    if (t >= M) return;
+    float acc = 0.0f;
+    for (int c = 0; c < S; ++c) acc += part[(size_t)c * M + t];
+    if (t < K * N) gW[t] = acc;
+    else gb[t - K * N] = acc;
 }
 
 // Adam (Kingma & Ba 2015), bias-corrected, on np parameters.
