@@ -10,7 +10,8 @@ pyglasstools/irvingkirkwood/IrvingKirkwood.h:74-110 and the functions it
 calls; each C function cites the reference lines it follows).  This module is
 the ctypes binding.
 
-Parity pins (tests/test_oracle_*.py):
+Parity pins (tests/test_oracle_*.py; the surrogate network's checker oracle/mlp.py
+is pinned in tests/test_surrogate_cpu.py):
   gl_nodes        numpy.polynomial.legendre.leggauss and 40-digit mpmath roots
   cg_delta        closed-form values, 2D normalisation integral, support
   bond_fixed      exact polynomial integral on inside segments (GL-n is exact
