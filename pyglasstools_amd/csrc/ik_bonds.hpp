@@ -177,7 +177,7 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
     constexpr int NB = 8;
     static_assert(MAXC >= NB, "list shorter than a batch");
     const int cx = cell_coord(g, 0, xi), cy = cell_coord(g, 1, yi), cz = cell_coord(g, 2, zi);
-    int rows[9];
+    int rows[kMaxRows];
     const int nrows = neighbour_rows(g, cy, cz, rows);
     int m = 0;
     for (int r = 0; r < nrows; ++r) {

@@ -65,17 +65,18 @@ __device__ __forceinline__ void load_entry(const ER *__restrict__ p, ER (&v)[ES]
     }
 }
 
-// Cells along one axis of the cell grid that cover cells c_j - 1 .. c_j + 1
+// Cells along one axis of the cell grid that cover cells c_j - R .. c_j + R
 // of PL points (point 0 always valid, the others where v[j]): `lo` (in
 // [0, nc)) and `cnt`, taken periodically; the whole axis when cnt would reach
-// nc (so no cell is listed twice).  Axes with fewer than 3 cells are
-// non-periodic or too short for a window and are taken whole.
+// nc (so no cell is listed twice).  An axis of one cell is non-periodic or
+// too short for a window.
 template <int PL>
-__device__ __forceinline__ void axis_window(int nc, const int (&c)[PL], const bool (&v)[PL], int &lo, int &cnt)
+__device__ __forceinline__ void axis_window(int nc, int R, const int (&c)[PL], const bool (&v)[PL], int &lo,
+                                            int &cnt)
 {
-    if (nc < 3) {
+    if (nc == 1) {
         lo = 0;
-        cnt = nc;
+        cnt = 1;
         return;
     }
     int dmin = 0, dmax = 0;
@@ -88,13 +89,13 @@ __device__ __forceinline__ void axis_window(int nc, const int (&c)[PL], const bo
         dmin = min(dmin, d);
         dmax = max(dmax, d);
     }
-    cnt = dmax - dmin + 3;
+    cnt = dmax - dmin + 2 * R + 1;
     if (cnt >= nc) {
         lo = 0;
         cnt = nc;
         return;
     }
-    lo = c[0] + dmin - 1;
+    lo = c[0] + dmin - R;
     if (lo < 0) lo += nc;
     else if (lo >= nc) lo -= nc;
 }
@@ -362,7 +363,7 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                 cc[2][j] = cell_coord(g, 2, xg[j][2]);
             }
 #pragma unroll
-            for (int a = 0; a < 3; ++a) axis_window<PL>(g.nc[a], cc[a], pv, wlo[a], wn[a]);
+            for (int a = 0; a < 3; ++a) axis_window<PL>(g.nc[a], g.reach, cc[a], pv, wlo[a], wn[a]);
         }
         const int nx = g.nc[0], ny = g.nc[1], nz = g.nc[2];
         int cnt = 0;

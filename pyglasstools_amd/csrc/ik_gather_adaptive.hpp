@@ -54,7 +54,7 @@ __global__ __launch_bounds__(128) void k_gather_adaptive(Geom g, int n, int G, c
         const double x0 = grid[(size_t)gp * 3], x1 = grid[(size_t)gp * 3 + 1];
         const double x2 = (D == 3) ? grid[(size_t)gp * 3 + 2] : 0.0;
         const int cx = cell_coord(g, 0, x0), cy = cell_coord(g, 1, x1), cz = cell_coord(g, 2, x2);
-        int rows[9];
+        int rows[kMaxRows];
         const int nrows = neighbour_rows(g, cy, cz, rows);
         for (int r = 0; r < nrows; ++r) {
             int rb[2], re[2];

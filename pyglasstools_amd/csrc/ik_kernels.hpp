@@ -20,9 +20,10 @@ namespace pgt {
 struct Geom {
     double L[3], invL[3], lo[3];
     int periodic[3];
-    int nc[3];     // cells per axis (1 on non-periodic axes and when < 3 fit)
-    double cs[3];  // cell edge
+    int nc[3];     // cells per axis: 1 (non-periodic axes, short boxes) or at least 2 reach + 1
+    double cs[3];  // cell edge, at least (search radius) / reach
     int ncell;
+    int reach;     // cells on each side of a cell that a search covers
     double rc2;      // cg_rcut^2
     double inv_rc;   // 1/cg_rcut
     double inv_rc2;  // 1/cg_rcut^2
@@ -47,47 +48,52 @@ __device__ __forceinline__ int cell_coord(const Geom &g, int a, double p)
     return k < 0 ? 0 : (k >= g.nc[a] ? g.nc[a] - 1 : k);
 }
 
-// Up to two contiguous particle ranges covering cells [cx-1, cx+1] of one
-// row of the cell grid along x (wrapping periodically).
+constexpr int kMaxRows = 25;  // rows of cells in a search window, (2 reach + 1)^2 with reach <= 2
+
+// Up to two contiguous particle ranges covering cells [cx - reach, cx + reach]
+// of one row of the cell grid along x (wrapping periodically).
 __device__ __forceinline__ int row_ranges(const Geom &g, const int *__restrict__ cstart, int rowbase, int cx,
                                           int *beg, int *end)
 {
-    const int nx = g.nc[0];
-    if (nx < 3) {
+    const int nx = g.nc[0], R = g.reach;
+    if (nx == 1) {
         beg[0] = cstart[rowbase];
-        end[0] = cstart[rowbase + nx];
+        end[0] = cstart[rowbase + 1];
         return 1;
     }
-    const int lo = cx - 1, hi = cx + 1;
+    const int lo = cx - R, hi = cx + R;
     if (lo >= 0 && hi < nx) {
         beg[0] = cstart[rowbase + lo];
         end[0] = cstart[rowbase + hi + 1];
         return 1;
     }
-    if (lo < 0) {  // cells nx-1 and [0, 1]
-        beg[0] = cstart[rowbase + nx - 1];
+    if (lo < 0) {  // cells [lo + nx, nx) and [0, hi]
+        beg[0] = cstart[rowbase + lo + nx];
         end[0] = cstart[rowbase + nx];
         beg[1] = cstart[rowbase];
-        end[1] = cstart[rowbase + 2];
+        end[1] = cstart[rowbase + hi + 1];
         return 2;
     }
-    // hi == nx: cells [nx-2, nx-1] and 0
-    beg[0] = cstart[rowbase + nx - 2];
+    // hi >= nx: cells [lo, nx) and [0, hi - nx]
+    beg[0] = cstart[rowbase + lo];
     end[0] = cstart[rowbase + nx];
     beg[1] = cstart[rowbase];
-    end[1] = cstart[rowbase + 1];
+    end[1] = cstart[rowbase + hi - nx + 1];
     return 2;
 }
 
-// Base indices of the (up to 9) rows of cells around cell (cy, cz).
+// Base indices of the (up to kMaxRows) rows of cells within reach of cell
+// (cy, cz), z outer, both ascending from -reach.
 __device__ __forceinline__ int neighbour_rows(const Geom &g, int cy, int cz, int *rowbase)
 {
-    const int ny = g.nc[1], nz = g.nc[2];
-    int ys[3], zs[3], nyy, nzz;
-    if (ny >= 3) { ys[0] = (cy + ny - 1) % ny; ys[1] = cy; ys[2] = (cy + 1) % ny; nyy = 3; }
-    else { for (int k = 0; k < ny; ++k) ys[k] = k; nyy = ny; }
-    if (nz >= 3) { zs[0] = (cz + nz - 1) % nz; zs[1] = cz; zs[2] = (cz + 1) % nz; nzz = 3; }
-    else { for (int k = 0; k < nz; ++k) zs[k] = k; nzz = nz; }
+    const int ny = g.nc[1], nz = g.nc[2], R = g.reach;
+    int ys[5], zs[5], nyy = 0, nzz = 0;
+    if (ny == 1) ys[nyy++] = 0;
+    else
+        for (int k = -R; k <= R; ++k) ys[nyy++] = (cy + k + ny) % ny;
+    if (nz == 1) zs[nzz++] = 0;
+    else
+        for (int k = -R; k <= R; ++k) zs[nzz++] = (cz + k + nz) % nz;
     int n = 0;
     for (int a = 0; a < nzz; ++a)
         for (int b = 0; b < nyy; ++b) rowbase[n++] = (zs[a] * ny + ys[b]) * g.nc[0];

@@ -264,7 +264,7 @@ __device__ __forceinline__ void for_ball(const Geom &g, const int *__restrict__ 
                                          double x0, double x1, F &&visit)
 {
     const int cx = cell_coord(g, 0, x0), cy = cell_coord(g, 1, x1);
-    int rows[9];
+    int rows[kMaxRows];
     const int nrows = neighbour_rows(g, cy, 0, rows);
     for (int r = 0; r < nrows; ++r) {
         int rb[2], re[2];
