@@ -540,13 +540,13 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
         const int *keys = kb[npass & 1]->p, *vals = vb[npass & 1]->p;
         const dim3 pgrid((unsigned)((n + 255) / 256), (unsigned)frame_rows(nf));
         if (D == 2)
-            hipLaunchKernelGGL((k_sort_finish<2, 2>), pgrid, dim3(256), 0, s, n, nf, g.ncell, keys, vals, pos, diam,
+            hipLaunchKernelGGL((k_sort_finish<2, 2>), pgrid, dim3(256), 0, s, g, n, nf, g.ncell, keys, vals, pos, diam,
                                mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
         else if (OD == 2)
-            hipLaunchKernelGGL((k_sort_finish<3, 2>), pgrid, dim3(256), 0, s, n, nf, g.ncell, keys, vals, pos, diam,
+            hipLaunchKernelGGL((k_sort_finish<3, 2>), pgrid, dim3(256), 0, s, g, n, nf, g.ncell, keys, vals, pos, diam,
                                mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
         else
-            hipLaunchKernelGGL((k_sort_finish<3, 3>), pgrid, dim3(256), 0, s, n, nf, g.ncell, keys, vals, pos, diam,
+            hipLaunchKernelGGL((k_sort_finish<3, 3>), pgrid, dim3(256), 0, s, g, n, nf, g.ncell, keys, vals, pos, diam,
                                mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
     }
     HIPCHK(hipGetLastError());

@@ -281,9 +281,11 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(int n, int nf, int 
 // Write the cell-sorted particle arrays (slot k takes input particle
 // vals[k]) and the cell starts: cstart[c] = first slot of cell c, and
 // cstart[ncell] = n (keys are sorted, so slot k writes the starts of the
-// cells after its predecessor's up to its own).
+// cells after its predecessor's up to its own).  Positions are wrapped into
+// the box along periodic axes (x - L floor((x - lo) / L): unchanged, bit for
+// bit, inside the box), so a particle sits where its cell says.
 template <int D, int OD>
-__global__ __launch_bounds__(256) void k_sort_finish(int n, int nf, int ncell, const int *__restrict__ keys,
+__global__ __launch_bounds__(256) void k_sort_finish(Geom g, int n, int nf, int ncell, const int *__restrict__ keys,
                                                      const int *__restrict__ vals, const double *__restrict__ pos,
                                                      const double *__restrict__ diam, const double *__restrict__ mass,
                                                      const double *__restrict__ vel, const double *__restrict__ disp,
@@ -307,9 +309,12 @@ __global__ __launch_bounds__(256) void k_sort_finish(int n, int nf, int ncell, c
     const size_t fi = (size_t)f * n + vals[fk];
     const double *p = pos + fi * 3;
     double *sp = spos + fk * PS;
-    sp[0] = p[0];
-    sp[1] = p[1];
-    if (D == 3) { sp[2] = p[2]; sp[3] = 0.0; }
+    auto wrap = [&](int a) {
+        return g.periodic[a] ? p[a] - g.L[a] * floor((p[a] - g.lo[a]) * g.invL[a]) : p[a];
+    };
+    sp[0] = wrap(0);
+    sp[1] = wrap(1);
+    if (D == 3) { sp[2] = wrap(2); sp[3] = 0.0; }
     sdiam[fk] = diam[fi];
     if (skin) {
         const double m = mass ? mass[fi] : 1.0;
