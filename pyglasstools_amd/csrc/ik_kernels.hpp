@@ -144,7 +144,7 @@ __device__ __forceinline__ T wave_sum(T v)
 // do not depend on scheduling.  No global atomics are used.
 // ------------------------------------------------------------------------
 constexpr int kSortBlock = 256;                     // threads per workgroup
-constexpr int kSortRows = 16;                       // rows of kSortBlock items per chunk
+constexpr int kSortRows = 4;                        // rows of kSortBlock items per chunk
 constexpr int kSortChunk = kSortBlock * kSortRows;  // items per chunk (one workgroup)
 constexpr int kSortMaxRadix = 256;                  // digits of at most 8 bits
 
