@@ -392,21 +392,59 @@ def batched_sweep(args):
         for j in jobs:
             j["e"].finish()  # also checks the bond capacity over every replay
             nc_graph += j["e"].stats()["n_contrib"]
+        # (c) one graph whose systems run on K streams forked from the capture
+        # stream and joined back (the largest system paired with the smallest)
+        K = 4
+        side = [torch.cuda.Stream() for _ in range(K)]
+        order = sorted(range(len(jobs)), key=lambda i: jobs[i]["sy"]["N"])
+        lane = {}
+        for k in range(len(order)):
+            lane[order[k]] = k if k < K else 2 * K - 1 - k
+        for i, j in enumerate(jobs):
+            j["e"].set_stream(side[lane[i] % K].cuda_stream)
+            with torch.cuda.stream(side[lane[i] % K]):
+                j["run"]()
+        torch.cuda.synchronize()
+        g2 = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g2, stream=s):
+            for st_ in side:
+                st_.wait_stream(s)
+            for j in jobs:
+                sy = j["sy"]
+                j["e"].enqueue_frames(FR, sy["N"], sy["pos"], sy["diam"], sy["mass"], sy["vel"], None,
+                                      fields=("virial", "kinetic"), d_virial=j["ov"], d_kinetic=j["ok"])
+            for st_ in side:
+                s.wait_stream(st_)
+        g2.replay()
+        torch.cuda.synchronize()
+        for j in jobs:
+            j["e"].finish()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g2.replay()
+        torch.cuda.synchronize()
+        t_graph_k = (time.perf_counter() - t0) / args.steps
+        for j in jobs:
+            j["e"].finish()
             j["e"].set_stream(0)
         nframes = FR * len(jobs)
         results[prec] = {"frames_per_s_graph": nframes / t_graph, "frames_per_s_eager": nframes / t_eager,
+                         "frames_per_s_graph_%d_streams" % K: nframes / t_graph_k,
                          "contributions_per_s_graph": nc / args.steps / t_graph,
                          "tuning": {str(j["sy"]["N"]): j["tuning"] for j in jobs}}
-        log("%s sweep: graph %.1f frames/s, eager %.1f frames/s" % (prec, nframes / t_graph, nframes / t_eager))
-        del jobs, g
+        log("%s sweep: graph %.1f frames/s (%d streams: %.1f), eager %.1f frames/s" %
+            (prec, nframes / t_graph, K, nframes / t_graph_k, nframes / t_eager))
+        del jobs, g, g2
         torch.cuda.empty_cache()
-    line = {"metric": METRIC, "value": round(results["fp64"]["frames_per_s_graph"], 3), "unit": "frames/s",
+    best = results["fp64"]["frames_per_s_graph_4_streams"]
+    line = {"metric": METRIC, "value": round(best, 3), "unit": "frames/s",
             "n_gpus": 1, "steps": args.steps, "warmup": 1,
-            "ms_per_step": round(FR * len(C5_N) / results["fp64"]["frames_per_s_graph"] * 1e3, 4),
+            "ms_per_step": round(FR * len(C5_N) / best * 1e3, 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (seeded lattice+jitter polydisperse glasses, 8 systems x 125 frames resident in HBM)",
             "config": {"workload": "2D polydisperse12, 8 systems N in %s x 125 frames, 128x128 grid each, Octic "
-                                   "r_cg=2.0, GL5, virial + kinetic; one HIP graph per sweep" % list(C5_N),
+                                   "r_cg=2.0, GL5, virial + kinetic; one HIP graph per sweep, systems on 4 streams forked "
+                                   "inside it" % list(C5_N),
                        "frames_per_step": FR * len(C5_N), "grid": "128x128", "parallelism": "single GPU"},
             "roofline": None, "cpu_baseline": None,
             "extra": {"fp64": results["fp64"], "fp32_form": results["fp32"],

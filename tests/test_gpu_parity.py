@@ -488,11 +488,13 @@ def test_adaptive_fields_match_oracle(gpu, cg):
     assert_parity(got, ref, st, ("virial", "kinetic"), tol=1e-8)
 
 
-def test_graph_captured_batches_match_eager(gpu):
+@pytest.mark.parametrize("forked", [False, True])
+def test_graph_captured_batches_match_eager(gpu, forked):
     """Two systems of different N, each a few frames, captured into one HIP
     graph through enqueue_frames (no sync, no allocation, no memset nodes) and
     replayed three times with eager work in between: every replay must give
-    the eager compute_frames result bit for bit."""
+    the eager compute_frames result bit for bit.  forked: each system on its
+    own stream, forked from the capture stream and joined back."""
     import torch
 
     dev = torch.device("cuda:0")
@@ -514,16 +516,23 @@ def test_graph_captured_batches_match_eager(gpu):
         jobs.append(dict(e=e, N=N, nf=nf, pos=pos, diam=diam, vel=vel, ov=ov, ok=ok,
                          ref_v=ov.clone(), ref_k=ok.clone()))
     s = torch.cuda.Stream()
-    for j in jobs:
-        j["e"].set_stream(s.cuda_stream)
+    side = [torch.cuda.Stream() for _ in jobs] if forked else [s for _ in jobs]
+    for j, st in zip(jobs, side):
+        j["e"].set_stream(st.cuda_stream)
         j["ov"].zero_()
         j["ok"].zero_()
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g, stream=s):
+        if forked:
+            for st in side:
+                st.wait_stream(s)
         for j in jobs:
             j["e"].enqueue_frames(j["nf"], j["N"], j["pos"], j["diam"], None, j["vel"], None,
                                   fields=("virial", "kinetic"), d_virial=j["ov"], d_kinetic=j["ok"])
+        if forked:
+            for st in side:
+                s.wait_stream(st)
     for rep in range(3):
         for j in jobs:
             j["ov"].fill_(-1.0)
