@@ -131,8 +131,9 @@ def precision_form(args):
     from pyglasstools_amd.autotune import Autotuner
 
     if args.block and args.pl:
-        eng.set_tuning(args.block, F, args.pl)
-        tuned = {"threads_per_block": args.block, "frames_per_launch": F, "points_per_lane": args.pl}
+        eng.set_tuning(args.block, F, args.pl, args.reach)
+        tuned = {"threads_per_block": args.block, "frames_per_launch": F, "points_per_lane": args.pl,
+                 "cell_reach": args.reach}
     else:
         log("autotuning threads per workgroup x points per lane")
         tuned, _ = Autotuner().tune(eng, step, ("c3", N, G, F), F, fpl=(F,), sync=torch.cuda.synchronize)
@@ -473,6 +474,7 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="gather threads per workgroup (0: autotune)")
     ap.add_argument("--fpl", type=int, default=0, help="frames per launch (0: autotune)")
     ap.add_argument("--pl", type=int, default=0, help="grid points per gather lane (0: autotune)")
+    ap.add_argument("--reach", type=int, default=1, help="cell reach with --block/--fpl/--pl fixed (1 or 2)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (reference dispatches for HBM byte counters) and exit")
     args = ap.parse_args()
@@ -539,10 +541,11 @@ def main():
 
     tuned, trials = None, []
     if args.block and args.fpl and args.pl:
-        eng.set_tuning(args.block, args.fpl, args.pl)
-        tuned = {"threads_per_block": args.block, "frames_per_launch": args.fpl, "points_per_lane": args.pl}
+        eng.set_tuning(args.block, args.fpl, args.pl, args.reach)
+        tuned = {"threads_per_block": args.block, "frames_per_launch": args.fpl, "points_per_lane": args.pl,
+                 "cell_reach": args.reach}
     else:
-        log("autotuning threads per workgroup x frames per launch x points per lane")
+        log("autotuning cell reach x points per lane x threads per workgroup x frames per launch")
         tuned, trials = Autotuner().tune(eng, step, ("c2", N, G, F, args.precision), F, sync=torch.cuda.synchronize)
         log("tuned: %s" % tuned)
 

@@ -180,11 +180,14 @@ int pgt_ik_finish(pgt_ik_engine *e);
 /* Run the engine's work on the given hipStream_t (NULL: the engine's own). */
 int pgt_ik_set_stream(pgt_ik_engine *e, void *hip_stream);
 
-/* Launch shape of the field gather: threads per workgroup (128 or 256),
- * frames per launch, and grid points per lane (1, 2 or 4: neighbouring
- * points that share one read of each particle and bond entry); 0 keeps the
+/* Launch shape: threads per workgroup of the field gather (128 or 256),
+ * frames per launch, grid points per gather lane (1, 2 or 4: neighbouring
+ * points that share one read of each particle and bond entry), and cell
+ * reach (1: cells of at least the search radius, searched one cell to each
+ * side; 2: cells of at least half of it, two to each side); 0 keeps the
  * current choice.  Used by the autotuner.  No counterpart in the reference. */
-int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_launch, int points_per_lane);
+int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_launch, int points_per_lane,
+                      int cell_reach);
 
 int pgt_ik_get_stats(pgt_ik_engine *e, pgt_ik_stats *out);
 

@@ -4,8 +4,10 @@ Three knobs of the engine are tuned per workload: threads per workgroup of
 the gather kernel (128 or 256; with the per-lane LDS lists this sets how many
 workgroups share a CU), frames per launch (how many frames one kernel launch
 covers; more frames amortise launch gaps, fewer keep the work arrays small)
-and grid points per gather lane (1, 2 or 4: more points share each read of a
-particle's bond entries but keep more registers and a longer LDS list).  Each candidate is timed on the caller's own job; the best is cached
+grid points per gather lane (1, 2 or 4: more points share each read of a
+particle's bond entries but keep more registers and a longer LDS list), and
+the cell reach (1 or 2: cells of the search radius searched one to each side,
+or of half of it searched two to each side).  Each candidate is timed on the caller's own job; the best is cached
 per workload key in memory and, when a path is given, in a JSON file.
 """
 import json
@@ -15,6 +17,8 @@ import time
 DEFAULT_BLOCK = (128, 256)
 DEFAULT_FPL = (4, 10, 25, 50, 100)
 DEFAULT_PL = (1, 2, 4)
+DEFAULT_REACH = (1, 2)
+KNOBS = ("threads_per_block", "frames_per_launch", "points_per_lane", "cell_reach")
 
 
 def _key(parts):
@@ -33,34 +37,36 @@ class Autotuner(object):
                 self.cache = {}
 
     def tune(self, engine, run_job, key, nframes, blocks=DEFAULT_BLOCK, fpl=DEFAULT_FPL, sync=None, repeats=2,
-             points=DEFAULT_PL):
+             points=DEFAULT_PL, reaches=DEFAULT_REACH):
         """run_job() runs the whole job once; sync() waits for the device.
         Returns (best settings dict, list of trial records)."""
         k = _key(key)
         if k in self.cache:
             best = self.cache[k]
-            engine.set_tuning(best["threads_per_block"], best["frames_per_launch"], best.get("points_per_lane", 1))
+            engine.set_tuning(best["threads_per_block"], best["frames_per_launch"], best.get("points_per_lane", 1),
+                              best.get("cell_reach", 1))
             return best, []
         trials = []
-        for pl in points:
-            for b in blocks:
-                for fp in sorted(set(min(f, nframes) for f in fpl)):
-                    engine.set_tuning(b, fp, pl)
-                    run_job()  # first call allocates for this shape
-                    if sync:
-                        sync()
-                    t = []
-                    for _ in range(repeats):
-                        t0 = time.perf_counter()
-                        run_job()
+        for rc in reaches:
+            for pl in points:
+                for b in blocks:
+                    for fp in sorted(set(min(f, nframes) for f in fpl)):
+                        engine.set_tuning(b, fp, pl, rc)
+                        run_job()  # first call allocates for this shape
                         if sync:
                             sync()
-                        t.append(time.perf_counter() - t0)
-                    trials.append({"threads_per_block": b, "frames_per_launch": fp, "points_per_lane": pl,
-                                   "seconds": min(t)})
+                        t = []
+                        for _ in range(repeats):
+                            t0 = time.perf_counter()
+                            run_job()
+                            if sync:
+                                sync()
+                            t.append(time.perf_counter() - t0)
+                        trials.append({"threads_per_block": b, "frames_per_launch": fp, "points_per_lane": pl,
+                                       "cell_reach": rc, "seconds": min(t)})
         best = min(trials, key=lambda r: r["seconds"])
-        best = {k: best[k] for k in ("threads_per_block", "frames_per_launch", "points_per_lane")}
-        engine.set_tuning(best["threads_per_block"], best["frames_per_launch"], best["points_per_lane"])
+        best = {k: best[k] for k in KNOBS}
+        engine.set_tuning(*[best[k] for k in KNOBS])
         self.cache[k] = best
         if self.cache_path:
             try:

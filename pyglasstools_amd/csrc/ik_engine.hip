@@ -251,6 +251,7 @@ struct pgt_ik_engine {
     int block = 256;
     int frames_per_launch = 8;
     int points_per_lane = 2;  // grid points per gather lane (Octic 5-node rule; 1 elsewhere)
+    int cell_reach = 1;       // cells searched on each side (cell edge >= search radius / reach)
     pgt_ik_stats stats{};
 };
 
@@ -319,11 +320,10 @@ Geom make_geom(const pgt_ik_engine *e, int D)
 {
     Geom g{};
     const double rmax = std::max(e->prm.cg_rcut, e->prm.max_rcut);
-    // cells of at least the search radius, windows of one cell each side
-    // (cells of half the radius with two each side scan 30% fewer candidates
-    // in 2D but measured slower: the candidate reads are not what bounds the
-    // searches, and the sort then needs 8-bit digits)
-    const int R = 1;
+    // cells of at least search radius / R, windows of R cells each side
+    // (R = 2 scans 30% fewer candidates in 2D, 42% in 3D, for 4x / 8x the
+    // cells; which is faster depends on the workload, so it is tuned)
+    const int R = e->cell_reach;
     g.reach = R;
     g.ncell = 1;
     for (int a = 0; a < 3; ++a) {
@@ -1058,7 +1058,8 @@ int pgt_ik_set_stream(pgt_ik_engine *e, void *hip_stream)
     return PGT_OK;
 }
 
-int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_launch, int points_per_lane)
+int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_launch, int points_per_lane,
+                      int cell_reach)
 {
     if (int rc = check_engine(e)) return rc;
     if (threads_per_block) {
@@ -1074,6 +1075,10 @@ int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_la
         if (points_per_lane != 1 && points_per_lane != 2 && points_per_lane != 4)
             return fail(PGT_ERR_ARG, "grid points per lane must be 1, 2 or 4");
         e->points_per_lane = points_per_lane;
+    }
+    if (cell_reach) {
+        if (cell_reach != 1 && cell_reach != 2) return fail(PGT_ERR_ARG, "cell reach must be 1 or 2");
+        e->cell_reach = cell_reach;
     }
     return PGT_OK;
 }

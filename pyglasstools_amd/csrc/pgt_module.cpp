@@ -542,7 +542,7 @@ struct IrvingKirkwood : Calculator {
         p.max_rcut = max_rcut;
         check(pgt_ik_create(&p, &engine), "IrvingKirkwood");
         engine_dim = dim;
-        if (threads_per_block) check(pgt_ik_set_tuning(engine, threads_per_block, 0, 0), "IrvingKirkwood");
+        if (threads_per_block) check(pgt_ik_set_tuning(engine, threads_per_block, 0, 0, 0), "IrvingKirkwood");
     }
     void compute(const darray &grid)
     {
@@ -762,7 +762,7 @@ PYBIND11_MODULE(_pgt, root)
         .def("printDisplacement", &IrvingKirkwood::printDisplacement)
         // additions beyond the reference surface
         .def("setPrecision", &IrvingKirkwood::setPrecision)
-        .def("setThreadsPerBlock", [](IrvingKirkwood &k, int b) { k.threads_per_block = b; if (k.engine) check(pgt_ik_set_tuning(k.engine, b, 0, 0), "setThreadsPerBlock"); })
+        .def("setThreadsPerBlock", [](IrvingKirkwood &k, int b) { k.threads_per_block = b; if (k.engine) check(pgt_ik_set_tuning(k.engine, b, 0, 0, 0), "setThreadsPerBlock"); })
         .def("getStats", &IrvingKirkwood::getStats)
         .def_readonly("max_rcut", &IrvingKirkwood::max_rcut);
     py::class_<CoarseGrainedField, Observable, std::shared_ptr<CoarseGrainedField>>(mi, "CoarseGrainedField");

@@ -90,7 +90,7 @@ def lib():
         L.pgt_ik_enqueue_frames.argtypes = [vp, i, i64, vp, vp, vp, vp, vp, ctypes.c_uint, vp, vp, vp]
         L.pgt_ik_finish.argtypes = [vp]
         L.pgt_ik_set_stream.argtypes = [vp, vp]
-        L.pgt_ik_set_tuning.argtypes = [vp, i, i, i]
+        L.pgt_ik_set_tuning.argtypes = [vp, i, i, i, i]
         L.pgt_ik_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.pgt_eval_pair_force.argtypes = [i, vp, i, d, i64, vp, vp, vp, vp, vp, vp]
         L.pgt_eval_cgfunc.argtypes = [i, d, i, i, i, d, d, i, i64, vp, vp, vp, vp, vp]
@@ -277,8 +277,8 @@ class Engine(object):
     def finish(self):
         _check(lib().pgt_ik_finish(self.h), "pgt_ik_finish")
 
-    def set_tuning(self, threads_per_block=0, frames_per_launch=0, points_per_lane=0):
-        _check(lib().pgt_ik_set_tuning(self.h, threads_per_block, frames_per_launch, points_per_lane),
+    def set_tuning(self, threads_per_block=0, frames_per_launch=0, points_per_lane=0, cell_reach=0):
+        _check(lib().pgt_ik_set_tuning(self.h, threads_per_block, frames_per_launch, points_per_lane, cell_reach),
                "pgt_ik_set_tuning")
 
     def stats(self):

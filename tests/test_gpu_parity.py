@@ -20,7 +20,7 @@ TOL = 1e-10
 
 
 def run_both(sysd, grid, fields=("virial",), dim=None, order=5, cg="octic", cg_rcut=2.0, pot="poly12", params=None,
-             scaled_rcut=1.25, max_rcut=None, block=None, precision="fp64", pl=None):
+             scaled_rcut=1.25, max_rcut=None, block=None, precision="fp64", pl=None, reach=None):
     dim = dim or sysd["dim"]
     params = params or oracle.poly12_params(rcut=scaled_rcut)
     if max_rcut is None:
@@ -32,6 +32,8 @@ def run_both(sysd, grid, fields=("virial",), dim=None, order=5, cg="octic", cg_r
         e.set_tuning(threads_per_block=block)
     if pl:
         e.set_tuning(points_per_lane=pl)
+    if reach:
+        e.set_tuning(cell_reach=reach)
     e.set_system(sysd, grid)
     got = e.compute(fields=fields)
     st = e.stats()
@@ -127,6 +129,27 @@ def test_points_per_lane_3d_elastic_fp32(gpu, pl):
     rel = np.abs(got["virial"] - ref["virial"]).max() / np.abs(ref["virial"]).max()
     print("fp32 form, %d points per lane: max|err|/max|T| = %.3e" % (pl, rel))
     assert rel < 1e-5
+
+
+@pytest.mark.parametrize("pl", [1, 2, 4])
+def test_cell_reach_two(gpu, pl):
+    """Cells of half the search radius, searched two cells to each side: the
+    same fields and counts, in 2D, 3D, the FP32 form and boxes of 5 to 7
+    cells per axis (windows that cover the whole axis)."""
+    sysd, grid = workload.config_frame(1, 2)
+    got, ref, st = run_both(sysd, grid, fields=("virial", "kinetic"), pl=pl, reach=2)
+    assert_parity(got, ref, st, ("virial", "kinetic"))
+    s3 = workload.make_system(3000, 3, seed=79)
+    got, ref, st = run_both(s3, workload.cartesian_grid_3d(s3["L"][0], 9), fields=("virial", "kinetic"), pl=pl,
+                            reach=2)
+    assert_parity(got, ref, st, ("virial", "kinetic"))
+    for n in (49, 81):
+        small = workload.make_system(n, 2, seed=60 + n)
+        got, ref, st = run_both(small, workload.cartesian_grid_2d(small["L"][0], 12), fields=("virial",), pl=pl,
+                                reach=2)
+        assert_parity(got, ref, st, ("virial",))
+    got, ref, st = run_both(sysd, grid, fields=("virial",), precision="fp32", pl=pl, reach=2)
+    assert np.abs(got["virial"] - ref["virial"]).max() < 1e-5 * np.abs(ref["virial"]).max()
 
 
 def test_points_per_lane_ignored_off_the_unrolled_path(gpu):

@@ -279,8 +279,8 @@ def test_interrupt_completes_the_frame_then_stops(tmp_path):
 
 
 def test_autotuner_picks_fastest_shape_and_caches(tmp_path):
-    """The autotuner times every (points per lane, threads, frames per launch)
-    shape on the caller's job, keeps the fastest, and replays it from its
+    """The autotuner times every (cell reach, points per lane, threads, frames
+    per launch) shape on the caller's job, keeps the fastest, and replays it from its
     cache (in memory and on disk) without timing again."""
     from pyglasstools_amd.autotune import Autotuner
 
@@ -289,12 +289,12 @@ def test_autotuner_picks_fastest_shape_and_caches(tmp_path):
             self.calls = []
             self.shape = None
 
-        def set_tuning(self, threads_per_block=0, frames_per_launch=0, points_per_lane=0):
-            self.calls.append((threads_per_block, frames_per_launch, points_per_lane))
-            self.shape = (threads_per_block, frames_per_launch, points_per_lane)
+        def set_tuning(self, threads_per_block=0, frames_per_launch=0, points_per_lane=0, cell_reach=0):
+            self.calls.append((threads_per_block, frames_per_launch, points_per_lane, cell_reach))
+            self.shape = (threads_per_block, frames_per_launch, points_per_lane, cell_reach)
 
     eng = FakeEngine()
-    cost = {(128, 10, 2): 1e-4}
+    cost = {(128, 10, 2, 2): 1e-4}
 
     def job():
         import time
@@ -302,10 +302,10 @@ def test_autotuner_picks_fastest_shape_and_caches(tmp_path):
 
     path = str(tmp_path / "tune.json")
     best, trials = Autotuner(path).tune(eng, job, ("k", 1), 40, fpl=(10, 100), repeats=1)
-    assert best == {"threads_per_block": 128, "frames_per_launch": 10, "points_per_lane": 2}
-    # 3 points-per-lane x 2 blocks x 2 frame counts (100 is capped at the 40 frames)
-    assert len(trials) == 12 and {t["frames_per_launch"] for t in trials} == {10, 40}
-    assert eng.shape == (128, 10, 2)
+    assert best == {"threads_per_block": 128, "frames_per_launch": 10, "points_per_lane": 2, "cell_reach": 2}
+    # 2 reaches x 3 points-per-lane x 2 blocks x 2 frame counts (100 is capped at the 40 frames)
+    assert len(trials) == 24 and {t["frames_per_launch"] for t in trials} == {10, 40}
+    assert eng.shape == (128, 10, 2, 2)
     eng2 = FakeEngine()
     best2, trials2 = Autotuner(path).tune(eng2, job, ("k", 1), 40)
-    assert best2 == best and trials2 == [] and eng2.calls == [(128, 10, 2)]
+    assert best2 == best and trials2 == [] and eng2.calls == [(128, 10, 2, 2)]
