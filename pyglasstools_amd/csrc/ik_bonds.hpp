@@ -25,12 +25,14 @@ namespace pgt {
 // gather's speed.
 //
 // FP32 (ER = float), one entry per read, padded to 16 bytes:
-//   [0, D) b', [D] C' = |r_ij|^2 / rc^2, [D+1] c', [D+2, ..) elastic
+//   2D: [0, 2) b', [2] C' = |r_ij|^2 / rc^2, [3] c', [4, ..) elastic
+//   3D: [0, 3) b', [3] c', [4, ..) elastic (|b'|^2 recomputed by the
+//   reader, so a virial entry is 16 bytes instead of 32)
 template <int D, int OD, bool ELA, typename ER>
 struct BondLayout {
     static constexpr int KN = OD * (OD + 1) / 2;
     static constexpr bool F64 = std::is_same<ER, double>::value;
-    static constexpr bool HAS_C2 = !F64;           // entry stores |b'|^2
+    static constexpr bool HAS_C2 = !F64 && D == 2;  // entry stores |b'|^2
     static constexpr int OC = HAS_C2 ? D + 1 : D;  // offset of c'
     static constexpr int OE = OC + 1;              // offset of the elastic terms
     static constexpr int RAW = OE + (ELA ? KN : 0);

@@ -427,6 +427,21 @@ def test_fp32_form_error_reported(gpu):
 # ---------------------------------------------------------------------------
 # scalar evaluators (they back getPairForceDivR / getDeltaFunc / getBondFunc)
 # ---------------------------------------------------------------------------
+@pytest.mark.parametrize("reach", [1, 2])
+def test_fp32_form_3d_error_reported(gpu, reach):
+    """The 3D FP32 form (16-byte bond entries, |b'|^2 recomputed in FP32):
+    per-component error against the FP64 oracle within 1e-5 of max|T|."""
+    s3 = workload.make_system(3000, 3, seed=81)
+    grid = workload.cartesian_grid_3d(s3["L"][0], 9)
+    got, ref, st = run_both(s3, grid, fields=("virial",), precision="fp32", reach=reach)
+    v, r = got["virial"], ref["virial"]
+    for a in range(3):
+        for b in range(a, 3):
+            rel = np.abs(v[:, b, a] - r[:, b, a]).max() / np.abs(r[:, b, a]).max()
+            print("3D fp32 form component %d%d: %.3e" % (a, b, rel))
+            assert rel < 1e-5
+
+
 def test_eval_pair_force_matches_oracle(gpu):
     rng = np.random.default_rng(12)
     n = 4000
