@@ -43,8 +43,8 @@ struct BondLayout {
     static_assert(GV % PER16 == 0, "a read group is a whole number of 16-byte pieces");
 };
 
-// One lane per particle.  Pass 1 walks the 3x3(x3) cells around the particle
-// (candidate positions loaded a batch at a time) and keeps, in a per-lane
+// One lane per particle.  Pass 1 walks the cells within reach of the particle's
+// cell (candidate positions loaded a batch at a time) and keeps, in a per-lane
 // LDS list, the candidates within max_rcut (about a third of them).  Pass 2
 // evaluates the pair force only for listed candidates, so the force code runs
 // with the lanes of a wavefront in step instead of behind a branch that some

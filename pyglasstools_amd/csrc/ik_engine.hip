@@ -2,20 +2,23 @@
 // the C ABI declared in include/pgt_ik.h.
 //
 // Per batch of frames the engine runs (kernels in ik_kernels.hpp):
-//   1. cell list    counting sort of the particles into cells of edge >= the
-//                   largest search radius; inside a cell particles keep their
-//                   input order, so every later sum has a fixed order
+//   1. cell list    stable radix sort of the particles by cell (cells of edge
+//                   >= the largest search radius / reach); inside a cell
+//                   particles keep their input order, so every later sum has
+//                   a fixed order
 //   2. bond list    for every particle i, the neighbours j within max_rcut
 //                   with a nonzero pair force, each stored once with the
 //                   scaled bond vector and the force coefficient: the force is
 //                   evaluated once per ordered pair instead of once per
 //                   (grid point, i, j) triple as in the reference loop
-//   3. field gather one lane per grid point: the particles inside the
-//                   coarse-graining ball are compacted into a per-lane LDS
-//                   list, then the lane integrates every bond of every listed
-//                   particle (5-node Gauss-Legendre on the Octic kernel by
-//                   default) and accumulates the tensors in FP64
-// Frames are stacked along blockIdx.y so one launch covers many frames.
+//   3. field gather one lane per 1, 2 or 4 grid points: the particles inside
+//                   the coarse-graining balls are compacted into a per-lane
+//                   LDS list, then the lane integrates every bond of every
+//                   listed particle (5-node Gauss-Legendre on the Octic kernel
+//                   by default) and accumulates the tensors in FP64
+// Frames are stacked along blockIdx.y so one launch covers many frames (the
+// cell-list kernels deal whole frames to XCDs, see frame_block).  The learned
+// bond-integral surrogate (ik_surrogate.hpp) has entry points of its own.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
