@@ -314,6 +314,23 @@ def test_bond_capacity_growth(gpu):
     assert_parity(got, ref, st, ("virial",))
 
 
+@pytest.mark.parametrize("pl", [1, 2, 4])
+def test_dense_cluster_unpacked_lists(gpu, pl):
+    """100 particles packed into a 1.3 x 1.3 patch: up to 99 bonds per
+    particle (a bond capacity above 63, so the gather keeps bond counts and
+    point masks in separate LDS arrays instead of packing them into the
+    index) and over 100 particles in some balls (the per-lane lists are
+    processed in several pieces)."""
+    sysd = workload.make_system(2000, 2, seed=19)
+    rng = np.random.default_rng(3)
+    sysd["pos"][:100, :2] = rng.uniform(-0.65, 0.65, (100, 2))
+    grid = workload.cartesian_grid_2d(sysd["L"][0], 24)
+    grid = np.concatenate([grid, np.array([[0.0, 0.0, 0.0], [0.3, -0.2, 0.0], [1.1, 0.4, 0.0]])])
+    got, ref, st = run_both(sysd, grid, fields=("virial", "kinetic"), pl=pl)
+    assert st["max_bonds"] > 63 and st["bond_capacity"] >= st["max_bonds"]
+    assert_parity(got, ref, st, ("virial", "kinetic"))
+
+
 def test_batched_frames_match_single(gpu):
     import torch
 
