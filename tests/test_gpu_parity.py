@@ -270,6 +270,17 @@ def test_positions_outside_the_box(gpu):
     assert np.abs(got["virial"] - base).max() <= 1e-12 * np.abs(base).max()
 
 
+@pytest.mark.parametrize("pl", [1, 2, 4])
+@pytest.mark.parametrize("periodic", [(0, 1, 0), (0, 0, 0)])
+def test_non_periodic_axes(gpu, pl, periodic):
+    """Open axes (one cell, no minimum image) with one or more points per lane."""
+    sysd = workload.make_system(800, 2, seed=23)
+    sysd["periodic"] = np.array(periodic)
+    grid = workload.cartesian_grid_2d(sysd["L"][0], 16)
+    got, ref, st = run_both(sysd, grid, fields=("virial", "kinetic"), pl=pl)
+    assert_parity(got, ref, st, ("virial", "kinetic"))
+
+
 def test_small_box_single_cell(gpu):
     """A box under three cell widths per axis uses one cell with minimum image."""
     sysd = workload.make_system(30, 2, seed=11)
