@@ -171,3 +171,30 @@ def test_trained_network_gain(gpu):
     none = e.compute_surrogate(p, b["inside"], b["crossing"], 0.5 * min(b["inside"], b["crossing"]))
     # the pair form sums in pair order, the gather in bond-count order
     assert e.stats()["n_network"] == 0 and np.abs(none - plain).max() <= 1e-13 * np.abs(plain).max()
+
+
+def test_surrogate_form_rejects_unsupported_settings(gpu):
+    """The surrogate stands in for the 2D Octic 5-node rule with FP64 bond
+    entries only: 3D systems, the FP32 form and other kernels or orders are
+    refused with PGT_ERR_UNSUPPORTED, and a missing frame with PGT_ERR_STATE."""
+    import torch
+
+    p = torch.tensor(surrogate.init_params(0), device=torch.device("cuda:0"))
+    sysd, grid = workload.config_frame(1, 0)
+    e = engine.Engine(dim=2, diameters=sysd["diam"])
+    with pytest.raises(engine.PGTError) as ex:
+        e.compute_surrogate(p, 0.1, 0.1, 1e-6)
+    assert ex.value.code == -3
+    s3 = workload.make_system(500, 3, seed=3)
+    cases = [(dict(dim=3), s3, workload.cartesian_grid_3d(s3["L"][0], 4)),
+             (dict(dim=2, precision="fp32"), sysd, grid),
+             (dict(dim=2, cg="rect"), sysd, grid),
+             (dict(dim=2, order=6), sysd, grid)]
+    for kw, sy, gr in cases:
+        e = engine.Engine(diameters=sy["diam"], **kw)
+        e.set_system(sy, gr)
+        with pytest.raises(engine.PGTError) as ex:
+            e.compute_surrogate(p, 0.1, 0.1, 1e-6)
+        assert ex.value.code == -5, kw
+        with pytest.raises(engine.PGTError):
+            e.bond_samples()
