@@ -24,22 +24,22 @@ namespace pgt {
 // scattered 16-byte piece per lane per cycle, so bytes per bond set the
 // gather's speed.
 //
-// FP32 (ER = float), one entry per read, padded to 16 bytes:
-//   2D: [0, 2) b', [2] C' = |r_ij|^2 / rc^2, [3] c', [4, ..) elastic
-//   3D: [0, 3) b', [3] c', [4, ..) elastic (|b'|^2 recomputed by the
-//   reader, so a virial entry is 16 bytes instead of 32)
+// FP32 (ER = float), the same fields unpadded (a 2D virial entry is 12
+// bytes, a 3D one 16), read in groups of 4, 2 or 1 entries (the fewest that
+// make a whole number of 16-byte pieces).
 template <int D, int OD, bool ELA, typename ER>
 struct BondLayout {
     static constexpr int KN = OD * (OD + 1) / 2;
     static constexpr bool F64 = std::is_same<ER, double>::value;
-    static constexpr bool HAS_C2 = !F64 && D == 2;  // entry stores |b'|^2
-    static constexpr int OC = HAS_C2 ? D + 1 : D;  // offset of c'
-    static constexpr int OE = OC + 1;              // offset of the elastic terms
+    static constexpr bool HAS_C2 = false;  // readers recompute |b'|^2
+    static constexpr int OC = D;           // offset of c'
+    static constexpr int OE = OC + 1;      // offset of the elastic terms
     static constexpr int RAW = OE + (ELA ? KN : 0);
     static constexpr int PER16 = 16 / (int)sizeof(ER);
-    static constexpr int ES = F64 ? RAW : ((RAW + PER16 - 1) / PER16) * PER16;
-    static constexpr int GRP = F64 ? 2 : 1;  // entries per read group
-    static constexpr int GV = GRP * ES;      // values per group, a multiple of PER16
+    static constexpr int ES = RAW;
+    // entries per read group (FP64 always pairs)
+    static constexpr int GRP = F64 ? 2 : (RAW % PER16 == 0 ? 1 : ((2 * RAW) % PER16 == 0 ? 2 : 4));
+    static constexpr int GV = GRP * ES;  // values per group, a multiple of PER16
     static_assert(GV % PER16 == 0, "a read group is a whole number of 16-byte pieces");
 };
 
@@ -77,9 +77,12 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
     ER *out = bonds + ((size_t)f * n + p) * (size_t)cap * ES;
     const double h = 0.5 * cgnorm;
     int cnt = 0;
-    ER pend[LY::RAW];  // FP64: an even entry awaiting its partner
+    constexpr int NP = LY::GRP > 1 ? LY::GRP - 1 : 1;
+    ER pend[NP][LY::RAW];  // entries awaiting the rest of their read group
 #pragma unroll
-    for (int k = 0; k < LY::RAW; ++k) pend[k] = ER(0);
+    for (int s = 0; s < NP; ++s)
+#pragma unroll
+        for (int k = 0; k < LY::RAW; ++k) pend[s][k] = ER(0);
     auto store_group = [](ER *dst, const ER (&gv)[LY::GV]) {
         using V = typename std::conditional<std::is_same<ER, double>::value, double2, float4>::type;
         V *q = reinterpret_cast<V *>(dst);
@@ -149,20 +152,25 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
                         if (OD == 2) t[2] = (ER)T(0, 1);
                         else { t[3] = (ER)T(0, 1); t[4] = (ER)T(0, 2); t[5] = (ER)T(1, 2); }
                     }
-                    if constexpr (LY::GRP == 2) {
-                        // FP64: an even entry waits in registers for its odd
-                        // partner, then both go out as 16-byte stores
-                        if ((cnt & 1) == 0) {
+                    if constexpr (LY::GRP > 1) {
+                        // the first GRP - 1 entries of a group wait in
+                        // registers; the last sends the group as 16-byte stores
+                        const int slot = cnt % LY::GRP;
+                        if (slot < LY::GRP - 1) {
 #pragma unroll
-                            for (int k = 0; k < LY::RAW; ++k) pend[k] = ev[k];
+                            for (int s = 0; s < NP; ++s)
+                                if (s == slot)
+#pragma unroll
+                                    for (int k = 0; k < LY::RAW; ++k) pend[s][k] = ev[k];
                         } else {
                             ER gv[LY::GV];
 #pragma unroll
-                            for (int k = 0; k < LY::RAW; ++k) {
-                                gv[k] = pend[k];
-                                gv[LY::ES + k] = ev[k];
-                            }
-                            store_group(out + (size_t)(cnt - 1) * ES, gv);
+                            for (int s = 0; s < LY::GRP - 1; ++s)
+#pragma unroll
+                                for (int k = 0; k < LY::RAW; ++k) gv[s * LY::ES + k] = pend[s][k];
+#pragma unroll
+                            for (int k = 0; k < LY::RAW; ++k) gv[(LY::GRP - 1) * LY::ES + k] = ev[k];
+                            store_group(out + (size_t)(cnt - (LY::GRP - 1)) * ES, gv);
                         }
                     } else {
                         ER *e = out + (size_t)cnt * ES;
@@ -213,12 +221,17 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
         }
     }
     if (m > 0) forces(m);
-    if constexpr (LY::GRP == 2)
-        if (live && (cnt & 1) && cnt <= cap) {  // the last, unpaired entry
-            ER *e = out + (size_t)(cnt - 1) * ES;
+    if constexpr (LY::GRP > 1) {
+        const int r = cnt % LY::GRP;  // entries of an unfinished last group
+        if (live && r && cnt <= cap)
 #pragma unroll
-            for (int k = 0; k < LY::RAW; ++k) e[k] = pend[k];
-        }
+            for (int s = 0; s < NP; ++s)
+                if (s < r) {
+                    ER *e = out + (size_t)(cnt - r + s) * ES;
+#pragma unroll
+                    for (int k = 0; k < LY::RAW; ++k) e[k] = pend[s][k];
+                }
+    }
 
     if (live) bcount[(size_t)f * n + p] = cnt;
     int mc = live ? cnt : 0;

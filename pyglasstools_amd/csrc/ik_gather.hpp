@@ -256,8 +256,12 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
         auto integrate_group = [&](const ER (&Gv)[GV], int k, int nb, int m, const ER (&ta)[PL][3],
                                    const ER (&Aj)[PL]) {
             integrate(Gv, std::integral_constant<int, 0>(), m, ta, Aj);
-            if constexpr (GRP == 2)
-                if (2 * k + 1 < nb) integrate(Gv, std::integral_constant<int, ES>(), m, ta, Aj);
+            if constexpr (GRP >= 2)
+                if (GRP * k + 1 < nb) integrate(Gv, std::integral_constant<int, ES>(), m, ta, Aj);
+            if constexpr (GRP >= 3)
+                if (GRP * k + 2 < nb) integrate(Gv, std::integral_constant<int, 2 * ES>(), m, ta, Aj);
+            if constexpr (GRP >= 4)
+                if (GRP * k + 3 < nb) integrate(Gv, std::integral_constant<int, 3 * ES>(), m, ta, Aj);
         };
 
         // pass B over the first cnt list entries
