@@ -325,6 +325,24 @@ def test_bond_capacity_growth(gpu):
     assert_parity(got, ref, st, ("virial",))
 
 
+def test_bond_capacity_growth_fp32(gpu):
+    """The same cluster in the 2D FP32 form (12-byte entries in groups of
+    four): the regrown capacity keeps groups whole, and each component stays
+    within 1e-5 of its max|T| against the FP64 oracle."""
+    sysd = workload.make_system(2000, 2, seed=9)
+    rng = np.random.default_rng(0)
+    sysd["pos"][:40, :2] = rng.uniform(-0.6, 0.6, (40, 2))
+    grid = workload.cartesian_grid_2d(sysd["L"][0], 16)
+    got, ref, st = run_both(sysd, grid, fields=("virial",), precision="fp32")
+    assert st["max_bonds"] > 16 and st["bond_capacity"] >= st["max_bonds"]
+    assert st["bond_capacity"] % 4 == 0
+    for a, b in ((0, 0), (1, 1), (0, 1)):
+        r = ref["virial"][:, b, a]
+        rel = np.abs(got["virial"][:, b, a] - r).max() / np.abs(r).max()
+        print("fp32 form, dense cluster, component %d%d: %.3e" % (a, b, rel))
+        assert rel < 1e-5
+
+
 @pytest.mark.parametrize("pl", [1, 2, 4])
 def test_dense_cluster_unpacked_lists(gpu, pl):
     """100 particles packed into a 1.3 x 1.3 patch: up to 99 bonds per
@@ -450,6 +468,26 @@ def test_fp32_form_error_reported(gpu):
         rel = d.max() / np.abs(ref_v[:, b, a]).max()
         print("fp32 form component %d%d: max|err|/max|T| = %.3e" % (a, b, rel))
         assert rel < 1e-5
+
+
+@pytest.mark.parametrize("pl", [1, 2, 4])
+def test_fp32_form_2d_all_fields(gpu, pl):
+    """The 2D FP32 form with virial, elastic and kinetic fields: 12-byte
+    virial entries read four at a time, 24-byte elastic entries read two at a
+    time, particles with bond counts that leave a partial last group.  Each
+    component within 1e-5 of its field's max|T| against the FP64 oracle."""
+    sysd, grid = workload.config_frame(1, 5)
+    sysd["disp"] = np.random.default_rng(4).normal(0, 0.05, (len(sysd["diam"]), 3))
+    sysd["disp"][:, 2] = 0
+    fields = ("virial", "elastic", "kinetic")
+    got, ref, st = run_both(sysd, grid, fields=fields, precision="fp32", pl=pl)
+    for name in fields:
+        for a, b in ((0, 0), (1, 1), (0, 1)):
+            r = ref[name][:, b, a]
+            rel = np.abs(got[name][:, b, a] - r).max() / np.abs(r).max()
+            print("fp32 form pl=%d %s %d%d: %.3e" % (pl, name, a, b, rel))
+            assert rel < 1e-5
+    assert st["n_ball"] == ref["n_ball"]
 
 
 # ---------------------------------------------------------------------------
