@@ -218,7 +218,9 @@ int pgt_mlp_infer(const float *d_params, int64_t n, const float *d_x, float *d_y
  * pair of IrvingKirkwood::compute's loop (IrvingKirkwood.h:80-107) with a
  * nonzero force, as input rows (d_x, 8 floats each) and its 5-node integral
  * (d_t).  *n_pairs gets the count; with d_x NULL only counts; PGT_ERR_RANGE if
- * max_pairs rows are too few.  Synchronises the engine's stream. */
+ * max_pairs rows are too few, or if the frame has more than 2^31 - 1 pairs
+ * (also refused by pgt_ik_compute_surrogate).  Synchronises the engine's
+ * stream. */
 int pgt_ik_bond_samples(pgt_ik_engine *e, int64_t max_pairs, float *d_x, float *d_t, int64_t *n_pairs);
 
 /* Virial field of the engine's frame (host output, G x 2 x 2) with the bond

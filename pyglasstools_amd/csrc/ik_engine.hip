@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -804,18 +805,21 @@ int prepare_pairs(pgt_ik_engine *e, int64_t *np, bool write, float *xin)
     const int G = (int)e->G;
     const Geom g = make_geom(e, 2);
     hipStream_t s = e->stream;
-    if ((rc = e->d_pcount.ensure(G)) || (rc = e->d_poff.ensure(G))) return rc;
+    // d_pn: [0] pairs taken from the network, [1] the 64-bit pair total
+    if ((rc = e->d_pcount.ensure(G)) || (rc = e->d_poff.ensure(G)) || (rc = e->d_pn.ensure(2))) return rc;
+    if ((rc = fill_async(e->d_pn.p + 1, 0ull, 1, s))) return rc;
     const dim3 gg((unsigned)((G + 127) / 128));
     hipLaunchKernelGGL(k_pair_count, gg, dim3(128), 0, s, g, G, e->d_grid.p, e->d_cstart.p, e->d_spos.p, e->cap,
-                       e->d_bcount.p, e->d_pcount.p);
+                       e->d_bcount.p, e->d_pcount.p, e->d_pn.p + 1);
+    unsigned long long total = 0;
+    HIPCHK(hipMemcpyAsync(&total, e->d_pn.p + 1, sizeof(total), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (total > (unsigned long long)INT_MAX)
+        return fail(PGT_ERR_RANGE, "the frame has more than 2^31 - 1 (grid point, bond) pairs; split the grid");
+    *np = (int64_t)total;
+    if (!write || *np == 0) return 0;
     HIPCHK(hipMemcpyAsync(e->d_poff.p, e->d_pcount.p, (size_t)G * sizeof(int), hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, G, e->d_poff.p);
-    int last[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(&last[0], e->d_poff.p + G - 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(&last[1], e->d_pcount.p + G - 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    *np = (int64_t)last[0] + last[1];
-    if (!write || *np == 0) return 0;
     if ((rc = e->d_prec.ensure((size_t)*np)) || (rc = e->d_pw.ensure((size_t)*np))) return rc;
     if (!xin) {
         if ((rc = e->d_pxin.ensure((size_t)*np * kMlpInPad))) return rc;
@@ -1394,7 +1398,7 @@ int pgt_ik_compute_surrogate(pgt_ik_engine *e, const float *d_params, double bou
     int rc;
     // events 0..2 belong to the bond-list job inside prepare_pairs
     if ((rc = ensure_events(e, 6))) return rc;
-    if ((rc = e->d_out.ensure((size_t)G * 4)) || (rc = e->d_pn.ensure(1))) return rc;
+    if ((rc = e->d_out.ensure((size_t)G * 4)) || (rc = e->d_pn.ensure(2))) return rc;
     hipStream_t s = e->stream;
     HIPCHK(hipEventRecord(e->ev[3], s));
     int64_t np = 0;

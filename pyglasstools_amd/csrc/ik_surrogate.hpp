@@ -280,16 +280,22 @@ __device__ __forceinline__ void for_ball(const Geom &g, const int *__restrict__ 
 }
 
 // Pairs of each grid point of one frame into counts[gp].
+// The 64-bit sum of all counts goes to *total (one atomic per wavefront), so
+// the host can refuse a frame whose pairs overflow the int offsets.
 __global__ __launch_bounds__(128) void k_pair_count(Geom g, int G, const double *__restrict__ grid,
                                                     const int *__restrict__ st, const double *__restrict__ sp, int cap,
-                                                    const int *__restrict__ bc, int *__restrict__ counts)
+                                                    const int *__restrict__ bc, int *__restrict__ counts,
+                                                    unsigned long long *__restrict__ total)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= G) return;
     int c = 0;
-    for_ball(g, st, sp, grid[(size_t)t * 3], grid[(size_t)t * 3 + 1],
-             [&](int p, double, double, double) { c += min(bc[p], cap); });
-    counts[t] = c;
+    if (t < G) {
+        for_ball(g, st, sp, grid[(size_t)t * 3], grid[(size_t)t * 3 + 1],
+                 [&](int p, double, double, double) { c += min(bc[p], cap); });
+        counts[t] = c;
+    }
+    const unsigned long long w = wave_sum((unsigned long long)c);
+    if ((threadIdx.x & 63) == 0 && w) atomicAdd(total, w);
 }
 
 // Write the pairs of grid point t at its offset (exclusive scan of the
