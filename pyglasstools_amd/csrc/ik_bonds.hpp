@@ -56,7 +56,7 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
                                                const int *__restrict__ cstart, const double *__restrict__ spos,
                                                const double *__restrict__ sdiam, const double *__restrict__ sdisp,
                                                int cap, int *__restrict__ bcount, ER *__restrict__ bonds,
-                                               int *__restrict__ wave_max)
+                                               int *__restrict__ wave_max, float4 *__restrict__ rec)
 {
     constexpr int PS = (D == 2) ? 2 : 4;
     using LY = BondLayout<D, OD, ELA, ER>;
@@ -233,7 +233,11 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
                 }
     }
 
-    if (live) bcount[(size_t)f * n + p] = cnt;
+    if (live) {
+        bcount[(size_t)f * n + p] = cnt;
+        // 3D: the record the gather's candidate scan reads (FP32 position, count)
+        if (D == 3) rec[(size_t)f * n + p] = make_float4((float)xi, (float)yi, (float)zi, __int_as_float(cnt));
+    }
     int mc = live ? cnt : 0;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) mc = max(mc, __shfl_xor(mc, off, 64));

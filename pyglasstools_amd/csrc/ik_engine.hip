@@ -233,6 +233,7 @@ struct pgt_ik_engine {
     bool have_particles = false, have_grid = false, have_disp = false, have_vel = false, have_mass = false;
     bool z_flat = true;       // all particle z are zero
     bool grid_z_flat = true;  // all grid z are zero
+    double grid_absmax = 0.0;  // largest |grid coordinate|
     DBuf<double> d_pos, d_diam, d_mass, d_vel, d_disp;  // single-frame host uploads
     DBuf<double> d_grid;
     DBuf<int> d_perm;
@@ -243,6 +244,7 @@ struct pgt_ik_engine {
     DBuf<double> d_gl;
     DBuf<unsigned long long> d_stats, d_wcounts;  // job totals; per-wavefront counts of a batch
     DBuf<int> d_wmax;                             // per-wavefront largest bond count of a batch
+    DBuf<float4> d_rec;                           // 3D, per sorted particle: FP32 position, bond count
     // surrogate field form: pair counts and offsets per grid point, pair
     // records, network inputs and outputs, final bond integrals, counter
     DBuf<int> d_pcount, d_poff;
@@ -345,6 +347,39 @@ Geom make_geom(const pgt_ik_engine *e, int D)
     g.inv_rc = 1.0 / e->prm.cg_rcut;
     g.inv_rc2 = 1.0 / g.rc2;
     g.mr2 = e->prm.max_rcut * e->prm.max_rcut;
+    // FP32 pre-test bounds (every axis searched must be periodic, so that
+    // particle coordinates are wrapped into the box).  Coordinates are at
+    // most M in magnitude; an FP32 component of a = r_i - x (difference,
+    // minimum image) is then within E = (4M + 2L + 4 r_cg) 2^-23 of the FP64
+    // one (twice the rounding bound), and near |a| = r_cg the FP32 |a|^2 is
+    // within D E (2.02 r_cg + 2E) + 3 2^-24 |a|^2 of the FP64 value; the
+    // band is twice that, and the thresholds are rounded outwards.
+    g.in32 = -1.0f;
+    g.out32 = INFINITY;
+    for (int a = 0; a < 3; ++a) {
+        g.L32[a] = (float)g.L[a];
+        g.invL32[a] = (float)g.invL[a];
+    }
+    bool ok = std::isfinite(e->grid_absmax);
+    double M = e->grid_absmax, Lm = 0.0, Lmin = INFINITY;
+    for (int a = 0; a < D; ++a) {
+        ok = ok && e->periodic[a];
+        M = std::max(M, std::max(std::fabs(e->lo[a]), std::fabs(e->lo[a] + e->L[a])));
+        Lm = std::max(Lm, e->L[a]);
+        Lmin = std::min(Lmin, e->L[a]);
+    }
+    const double rc = e->prm.cg_rcut;
+    const double Ee = (4 * M + 2 * Lm + 4 * rc) * std::ldexp(1.0, -23);
+    const double B = 2 * (D * Ee * (2.02 * rc + 2 * Ee) + 3 * std::ldexp(1.0, -24) * 1.03 * g.rc2);
+    // (a search radius well below half of every box length, so that the FP32
+    // and FP64 minimum images can differ only for candidates far outside)
+    if (ok && g.rc2 - B > 0 && 2 * rc + 8 * Ee < Lmin) {
+        float lo32 = (float)(g.rc2 - B), hi32 = (float)(g.rc2 + B);
+        if ((double)lo32 > g.rc2 - B) lo32 = std::nextafter(lo32, -INFINITY);
+        if ((double)hi32 < g.rc2 + B) hi32 = std::nextafter(hi32, INFINITY);
+        g.in32 = lo32;
+        g.out32 = hi32;
+    }
     return g;
 }
 
@@ -367,6 +402,7 @@ struct LaunchCtx {
     const int *cstart;
     const double *spos, *skin;
     const int *bcount;
+    const float4 *rec;  // FP32 position and bond count per sorted particle
     const void *bonds;
     GatherOut out;
     hipStream_t stream;
@@ -391,11 +427,11 @@ void launch_gather_t(const LaunchCtx &c)
     if (c.pack)
         hipLaunchKernelGGL((k_gather<D, OD, CG, NQ, VIR, ELA, ER, BLK, MAXL, true, PL>), grid, dim3(BLK), 0, c.stream,
                            c.g, c.n, c.G, c.grid, c.perm, c.R, c.gld, c.cgnorm, c.cstart, c.spos, c.skin, c.cap,
-                           c.bcount, (const ER *)c.bonds, c.out);
+                           c.bcount, c.rec, (const ER *)c.bonds, c.out);
     else
         hipLaunchKernelGGL((k_gather<D, OD, CG, NQ, VIR, ELA, ER, BLK, MAXL, false, PL>), grid, dim3(BLK), 0,
                            c.stream, c.g, c.n, c.G, c.grid, c.perm, c.R, c.gld, c.cgnorm, c.cstart, c.spos, c.skin,
-                           c.cap, c.bcount, (const ER *)c.bonds, c.out);
+                           c.cap, c.bcount, c.rec, (const ER *)c.bonds, c.out);
 }
 
 template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, typename ER, int BLK>
@@ -443,7 +479,7 @@ constexpr int kBondBlock = 128;  // k_bonds threads per workgroup
 
 template <int D, int OD, typename ER>
 void launch_bonds_t(const LaunchCtx &c, unsigned fields, const PotParams &P, const double *sdiam, const double *sdisp,
-                    int *bcount, void *bonds, int *wave_max)
+                    int *bcount, void *bonds, int *wave_max, float4 *rec)
 {
     const bool V = fields & PGT_FIELD_VIRIAL, E = fields & PGT_FIELD_ELASTIC;
     // candidate list per lane: ~2.5x the mean number of neighbours within
@@ -452,13 +488,13 @@ void launch_bonds_t(const LaunchCtx &c, unsigned fields, const PotParams &P, con
     dim3 grid((unsigned)((c.n + BLK - 1) / BLK), (unsigned)c.nf);
     if (V && E)
         hipLaunchKernelGGL((k_bonds<D, OD, true, true, ER, BLK, MAXC>), grid, dim3(BLK), 0, c.stream, c.g, c.n, P,
-                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, wave_max);
+                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, wave_max, rec);
     else if (E)
         hipLaunchKernelGGL((k_bonds<D, OD, false, true, ER, BLK, MAXC>), grid, dim3(BLK), 0, c.stream, c.g, c.n, P,
-                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, wave_max);
+                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, wave_max, rec);
     else
         hipLaunchKernelGGL((k_bonds<D, OD, true, false, ER, BLK, MAXC>), grid, dim3(BLK), 0, c.stream, c.g, c.n, P,
-                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, wave_max);
+                           c.cgnorm, c.cstart, c.spos, sdiam, sdisp, c.cap, bcount, (ER *)bonds, wave_max, rec);
 }
 
 template <int D, int OD, typename ER>
@@ -515,6 +551,7 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
         return rc;
     if (kin && (rc = need(e->d_skin, (size_t)nf * n * KN))) return rc;
     if (ela && (rc = need(e->d_sdisp, (size_t)nf * n * 3))) return rc;
+    if (pairs && D == 3 && (rc = need(e->d_rec, (size_t)nf * n))) return rc;
     hipStream_t s = e->stream;
     if (ev_begin) HIPCHK(hipEventRecord(ev_begin, s));
     double *skin = kin ? e->d_skin.p : nullptr;
@@ -597,19 +634,20 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
         c.cap = e->cap;
         const PotParams P = make_pot(e->prm);
         if (D == 2) {
-            if (fp32) launch_bonds_t<2, 2, float>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p);
-            else launch_bonds_t<2, 2, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p);
+            if (fp32) launch_bonds_t<2, 2, float>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p, e->d_rec.p);
+            else launch_bonds_t<2, 2, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p, e->d_rec.p);
         } else if (OD == 2) {
-            launch_bonds_t<3, 2, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p);
+            launch_bonds_t<3, 2, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p, e->d_rec.p);
         } else {
-            if (fp32) launch_bonds_t<3, 3, float>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p);
-            else launch_bonds_t<3, 3, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p);
+            if (fp32) launch_bonds_t<3, 3, float>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p, e->d_rec.p);
+            else launch_bonds_t<3, 3, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p, e->d_rec.p);
         }
         HIPCHK(hipGetLastError());
     } else {
         c.cap = 1;
     }
     c.bcount = e->d_bcount.p;
+    c.rec = pairs && D == 3 ? e->d_rec.p : nullptr;
     c.bonds = e->d_bonds.p;
     c.pack = n < (1 << (c.pl > 1 ? 22 : 26)) && c.cap <= 63;
     if (ev_gather) HIPCHK(hipEventRecord(ev_gather, s));
@@ -909,6 +947,7 @@ void pgt_ik_destroy(pgt_ik_engine *e)
     e->d_stats.release();
     e->d_wcounts.release();
     e->d_wmax.release();
+    e->d_rec.release();
     for (hipEvent_t v : e->ev) (void)hipEventDestroy(v);
     if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
     delete e;
@@ -977,7 +1016,12 @@ int pgt_ik_set_grid(pgt_ik_engine *e, int64_t g, const double *grid)
     if (g < 0 || g > (int64_t)1 << 30) return fail(PGT_ERR_ARG, "grid size out of range");
     if (g > 0 && !grid) return fail(PGT_ERR_ARG, "null grid");
     bool flat = true;
-    for (int64_t i = 0; i < g && flat; ++i) flat = grid[3 * i + 2] == 0.0;
+    double amax = 0.0;
+    for (int64_t i = 0; i < g; ++i) {
+        flat = flat && grid[3 * i + 2] == 0.0;
+        for (int a = 0; a < 3; ++a) amax = std::max(amax, std::fabs(grid[3 * i + a]));
+    }
+    if (!(amax < INFINITY)) amax = INFINITY;  // NaN coordinates: no FP32 pre-test
     const std::vector<int> perm = morton_order(g, grid, !flat || e->prm.dim == 3);
     int rc;
     if ((rc = e->d_grid.ensure((size_t)g * 3)) || (rc = e->d_perm.ensure((size_t)g))) return rc;
@@ -987,6 +1031,7 @@ int pgt_ik_set_grid(pgt_ik_engine *e, int64_t g, const double *grid)
     }
     HIPCHK(hipStreamSynchronize(e->stream));
     e->grid_z_flat = flat;
+    e->grid_absmax = amax;
     e->G = g;
     e->have_grid = true;
     return PGT_OK;

@@ -111,7 +111,12 @@ __device__ __forceinline__ void axis_window(int nc, int R, const int (&c)[PL], c
 //
 // Pass A scans the cells around the lane's points and compacts the particles
 // inside at least one ball, with their bond counts and the mask of the points
-// whose ball holds them, into a per-lane list in LDS.  With PACK one int per
+// whose ball holds them, into a per-lane list in LDS.  In 3D with bonds it
+// reads one 16-byte record per candidate (FP32 position and bond count,
+// written by the bond kernel) instead of 32 bytes of FP64 position and a
+// separate count, and tests |a|^2 in FP32 against thresholds that bound the
+// FP64 rounding (Geom::in32, out32); a candidate between them is decided by
+// the FP64 test, so the particles listed are exactly those of the FP64 scan.  With PACK one int per
 // entry holds index | count << IB | mask << (IB + 6) (IB = 26 for one point
 // per lane, 22 otherwise: fewer than 2^IB particles and at most 63 bonds
 // each); otherwise separate arrays hold the counts and masks.  The list is
@@ -128,8 +133,8 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                                                 const double *__restrict__ gld, double cgnorm,
                                                 const int *__restrict__ cstart, const double *__restrict__ spos,
                                                 const double *__restrict__ skin, int cap,
-                                                const int *__restrict__ bcount, const ER *__restrict__ bonds,
-                                                GatherOut out)
+                                                const int *__restrict__ bcount, const float4 *__restrict__ rec,
+                                                const ER *__restrict__ bonds, GatherOut out)
 {
     static_assert(PL == 1 || PL == 2 || PL == 4, "1, 2 or 4 grid points per lane");
     constexpr int PS = (D == 2) ? 2 : 4;
@@ -163,7 +168,12 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
     const bool KIN = skin != nullptr;
     const double *sp = spos + (size_t)f * n * PS;
     const int *st = cstart + (size_t)f * (g.ncell + 1);
+    // 3D with bonds: the candidate scan reads the bond kernel's 16-byte
+    // records (FP32 position, bond count) instead of the FP64 positions and
+    // the counts, and tests in FP32 first
+    constexpr bool RECS = PAIRS && D == 3;
     const int *bc = bcount + (size_t)f * n;
+    const float4 *rc4 = RECS ? rec + (size_t)f * n : nullptr;
     const ER *bd = bonds + (size_t)f * n * (size_t)cap * ES;
     const double *sk = KIN ? skin + (size_t)f * n * KN : nullptr;
 
@@ -193,6 +203,11 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
             xg[j][2] = (D == 3) ? grid[(size_t)q * 3 + 2] : 0.0;
         }
         const double s2 = 2.0 * g.inv_rc;
+        float xg32[PL][3];
+#pragma unroll
+        for (int j = 0; j < PL; ++j)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) xg32[j][a] = (float)xg[j][a];
 
         // integrate the entry at offset O (a std::integral_constant) of a
         // read group for the points of mask m
@@ -394,28 +409,69 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                             process(cnt);
                             cnt = 0;
                         }
-                        double q[NB][PS];
+                        // exact test: FP64 minimum image and |a|^2 against r_cg^2
+                        auto exact_mask = [&](const double *qq) {
+                            int mm = 0;
+#pragma unroll
+                            for (int j = 0; j < PL; ++j) {
+                                const double a0 = minimg(qq[0] - xg[j][0], g.L[0], g.invL[0], g.periodic[0]);
+                                const double a1 = minimg(qq[1] - xg[j][1], g.L[1], g.invL[1], g.periodic[1]);
+                                const double a2 =
+                                    (D == 3) ? minimg(qq[2] - xg[j][2], g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                                const double asq = a0 * a0 + a1 * a1 + a2 * a2;
+                                if (pv[j] && asq < g.rc2) mm |= 1 << j;
+                            }
+                            return mm;
+                        };
+                        double q[RECS ? 1 : NB][PS];
+                        float4 r4[RECS ? NB : 1];
                         int qb[NB];
 #pragma unroll
                         for (int u = 0; u < NB; ++u) {
                             const int p = min(p0 + u, re[s] - 1);
+                            if constexpr (RECS) {
+                                // one 16-byte record: FP32 position and bond count
+                                r4[u] = rc4[p];
+                                qb[u] = __float_as_int(r4[u].w);
+                            } else {
 #pragma unroll
-                            for (int a = 0; a < PS; ++a) q[u][a] = sp[(size_t)p * PS + a];
-                            if constexpr (PAIRS) qb[u] = bc[p];
+                                for (int a = 0; a < PS; ++a) q[u][a] = sp[(size_t)p * PS + a];
+                                if constexpr (PAIRS) qb[u] = bc[p];
+                            }
                         }
 #pragma unroll
                         for (int u = 0; u < NB; ++u) {
                             const int p = p0 + u;
                             if (p >= re[s]) break;
                             int m = 0;
+                            if constexpr (RECS) {
+                                // FP32 pre-test; a candidate in the band
+                                // around r_cg (or any, when the geometry gives
+                                // no bound) is decided by the exact test
+                                bool unsure = false;
 #pragma unroll
-                            for (int j = 0; j < PL; ++j) {
-                                const double a0 = minimg(q[u][0] - xg[j][0], g.L[0], g.invL[0], g.periodic[0]);
-                                const double a1 = minimg(q[u][1] - xg[j][1], g.L[1], g.invL[1], g.periodic[1]);
-                                const double a2 =
-                                    (D == 3) ? minimg(q[u][2] - xg[j][2], g.L[2], g.invL[2], g.periodic[2]) : 0.0;
-                                const double asq = a0 * a0 + a1 * a1 + a2 * a2;
-                                if (pv[j] && asq < g.rc2) m |= 1 << j;
+                                for (int j = 0; j < PL; ++j) {
+                                    float d0 = r4[u].x - xg32[j][0], d1 = r4[u].y - xg32[j][1], d2 = 0.0f;
+                                    if (g.periodic[0]) d0 = fmaf(-g.L32[0], rintf(d0 * g.invL32[0]), d0);
+                                    if (g.periodic[1]) d1 = fmaf(-g.L32[1], rintf(d1 * g.invL32[1]), d1);
+                                    if (D == 3) {
+                                        d2 = r4[u].z - xg32[j][2];
+                                        if (g.periodic[2]) d2 = fmaf(-g.L32[2], rintf(d2 * g.invL32[2]), d2);
+                                    }
+                                    const float a2f = d0 * d0 + d1 * d1 + d2 * d2;
+                                    if (pv[j]) {
+                                        if (a2f < g.in32) m |= 1 << j;
+                                        else if (!(a2f > g.out32)) unsure = true;
+                                    }
+                                }
+                                if (unsure) {
+                                    double qq[PS];
+#pragma unroll
+                                    for (int a = 0; a < PS; ++a) qq[a] = sp[(size_t)p * PS + a];
+                                    m = exact_mask(qq);
+                                }
+                            } else {
+                                m = exact_mask(q[u]);
                             }
                             if (!m) continue;
                             nball += PL == 1 ? 1 : __popc(m);

@@ -28,6 +28,12 @@ struct Geom {
     double inv_rc;   // 1/cg_rcut
     double inv_rc2;  // 1/cg_rcut^2
     double mr2;      // max_rcut^2
+    // FP32 pre-test of the gather's candidate scan: with |a|^2 from FP32
+    // copies of the positions, |a|^2 < in32 means inside the ball and
+    // |a|^2 > out32 outside, whatever the FP64 rounding; in between the FP64
+    // test decides.  in32 < 0 and out32 = inf send every candidate to it.
+    float L32[3], invL32[3];
+    float in32, out32;
 };
 
 // Minimum image with the image index rint(d/L) taken as rint(d * (1/L)).

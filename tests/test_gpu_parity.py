@@ -198,6 +198,36 @@ def test_3d_small(gpu):
     assert_parity(got, ref, st, ("virial", "kinetic"))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_3d_candidates_on_the_ball_surface(gpu, precision):
+    """The 3D candidate scan tests distances in FP32 first and leaves those
+    near r_cg to the FP64 test.  Grid points placed at r_cg (1 +- 1e-13 ...
+    1e-5) from particles, in random directions, put many candidates in that
+    band; which particles are inside a ball must still be exactly the
+    oracle's, as must the fields.  (Offsets stay well above the last bit of
+    |a|^2, where the FP64 sum itself may round differently with or without
+    fused multiply-adds.)"""
+    sysd = workload.make_system(3000, 3, seed=71)
+    rng = np.random.default_rng(8)
+    idx = rng.choice(len(sysd["diam"]), 160, replace=False)
+    u = rng.normal(size=(160, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    rel = np.array([-1e-5, -1e-7, -1e-10, -1e-13, 1e-13, 1e-10, 1e-7, 1e-5])
+    grid = (sysd["pos"][idx] + 2.0 * u * (1.0 + rel[np.arange(160) % len(rel)])[:, None])
+    # back into the box, as the particles are
+    L = sysd["L"]
+    grid = grid - L * np.floor((grid + L / 2) / L)
+    got, ref, st = run_both(sysd, grid, fields=("virial", "kinetic"), precision=precision)
+    assert st["n_ball"] == ref["n_ball"]
+    if precision == "fp64":
+        assert_parity(got, ref, st, ("virial", "kinetic"))
+    else:
+        for a in range(3):
+            for b in range(a, 3):
+                r = ref["virial"][:, b, a]
+                assert np.abs(got["virial"][:, b, a] - r).max() <= 1e-5 * np.abs(r).max()
+
+
 def test_2d_system_with_3d_fields_nonflat_z(gpu):
     """A 2D box whose particles sit off z=0 takes the general 3-component path."""
     sysd = workload.make_system(1500, 2, seed=5)
