@@ -74,6 +74,8 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
     const int *st = cstart + (size_t)f * (g.ncell + 1);
     const double xi = sp[(size_t)p * PS], yi = sp[(size_t)p * PS + 1], zi = (D == 3) ? sp[(size_t)p * PS + 2] : 0.0;
     const double di = sd[p];
+    const float4 *rf = D == 3 ? rec + (size_t)f * n : nullptr;  // 3D: FP32 records of this frame
+    const float xi32 = (float)xi, yi32 = (float)yi, zi32 = (float)zi;
     ER *out = bonds + ((size_t)f * n + p) * (size_t)cap * ES;
     const double h = 0.5 * cgnorm;
     int cnt = 0;
@@ -199,23 +201,58 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
                     forces(m);
                     m = 0;
                 }
-                double cq[NB][PS];
+                // exact test: FP64 minimum image and |r_ij|^2 against max_rcut^2
+                auto within = [&](const double *cq) {
+                    const double b0 = minimg(cq[0] - xi, g.L[0], g.invL[0], g.periodic[0]);
+                    const double b1 = minimg(cq[1] - yi, g.L[1], g.invL[1], g.periodic[1]);
+                    const double b2 = (D == 3) ? minimg(cq[2] - zi, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                    return b0 * b0 + b1 * b1 + b2 * b2 < g.mr2;
+                };
+                if constexpr (D == 3) {
+                    // FP32 positions from the records, tested against
+                    // bounds of the FP64 rounding; the band between them
+                    // goes to the exact test (see Geom::bin32)
+                    float c32[NB][3];
 #pragma unroll
-                for (int u = 0; u < NB; ++u) {
-                    const int q = min(q0 + u, re[s] - 1);
+                    for (int u = 0; u < NB; ++u) {
+                        const float *r = reinterpret_cast<const float *>(rf + min(q0 + u, re[s] - 1));
 #pragma unroll
-                    for (int a = 0; a < PS; ++a) cq[u][a] = sp[(size_t)q * PS + a];
-                }
+                        for (int a = 0; a < 3; ++a) c32[u][a] = r[a];
+                    }
 #pragma unroll
-                for (int u = 0; u < NB; ++u) {
-                    const int q = q0 + u;
-                    if (q >= re[s]) break;
-                    const double b0 = minimg(cq[u][0] - xi, g.L[0], g.invL[0], g.periodic[0]);
-                    const double b1 = minimg(cq[u][1] - yi, g.L[1], g.invL[1], g.periodic[1]);
-                    const double b2 = (D == 3) ? minimg(cq[u][2] - zi, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
-                    const double rsq = b0 * b0 + b1 * b1 + b2 * b2;
-                    if (q == p || !(rsq < g.mr2)) continue;
-                    lq[m++][lane] = q;
+                    for (int u = 0; u < NB; ++u) {
+                        const int q = q0 + u;
+                        if (q >= re[s]) break;
+                        float d0 = c32[u][0] - xi32, d1 = c32[u][1] - yi32, d2 = c32[u][2] - zi32;
+                        if (g.periodic[0]) d0 = fmaf(-g.L32[0], rintf(d0 * g.invL32[0]), d0);
+                        if (g.periodic[1]) d1 = fmaf(-g.L32[1], rintf(d1 * g.invL32[1]), d1);
+                        if (g.periodic[2]) d2 = fmaf(-g.L32[2], rintf(d2 * g.invL32[2]), d2);
+                        const float a2f = d0 * d0 + d1 * d1 + d2 * d2;
+                        bool in = a2f < g.bin32;
+                        if (!in && !(a2f > g.bout32)) {
+                            double cq[3];
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) cq[a] = sp[(size_t)q * PS + a];
+                            in = within(cq);
+                        }
+                        if (q == p || !in) continue;
+                        lq[m++][lane] = q;
+                    }
+                } else {
+                    double cq[NB][PS];
+#pragma unroll
+                    for (int u = 0; u < NB; ++u) {
+                        const int q = min(q0 + u, re[s] - 1);
+#pragma unroll
+                        for (int a = 0; a < PS; ++a) cq[u][a] = sp[(size_t)q * PS + a];
+                    }
+#pragma unroll
+                    for (int u = 0; u < NB; ++u) {
+                        const int q = q0 + u;
+                        if (q >= re[s]) break;
+                        if (q == p || !within(cq[u])) continue;
+                        lq[m++][lane] = q;
+                    }
                 }
             }
         }
@@ -235,8 +272,9 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
 
     if (live) {
         bcount[(size_t)f * n + p] = cnt;
-        // 3D: the record the gather's candidate scan reads (FP32 position, count)
-        if (D == 3) rec[(size_t)f * n + p] = make_float4((float)xi, (float)yi, (float)zi, __int_as_float(cnt));
+        // 3D: the bond count completes the particle's FP32 record (the
+        // candidate scans read only its position words)
+        if (D == 3) reinterpret_cast<int *>(rec + (size_t)f * n + p)[3] = cnt;
     }
     int mc = live ? cnt : 0;
 #pragma unroll

@@ -347,15 +347,15 @@ Geom make_geom(const pgt_ik_engine *e, int D)
     g.inv_rc = 1.0 / e->prm.cg_rcut;
     g.inv_rc2 = 1.0 / g.rc2;
     g.mr2 = e->prm.max_rcut * e->prm.max_rcut;
-    // FP32 pre-test bounds (every axis searched must be periodic, so that
-    // particle coordinates are wrapped into the box).  Coordinates are at
-    // most M in magnitude; an FP32 component of a = r_i - x (difference,
-    // minimum image) is then within E = (4M + 2L + 4 r_cg) 2^-23 of the FP64
-    // one (twice the rounding bound), and near |a| = r_cg the FP32 |a|^2 is
-    // within D E (2.02 r_cg + 2E) + 3 2^-24 |a|^2 of the FP64 value; the
-    // band is twice that, and the thresholds are rounded outwards.
-    g.in32 = -1.0f;
-    g.out32 = INFINITY;
+    // FP32 pre-test bounds for a search radius r (r_cg for the gather,
+    // max_rcut for the bond list); every axis searched must be periodic, so
+    // that particle coordinates are wrapped into the box.  Coordinates (box
+    // and grid) are at most M in magnitude; an FP32 component of a
+    // separation (difference, minimum image) is then within
+    // E = (4M + 2L + 4r) 2^-23 of the FP64 one (twice the rounding bound),
+    // and near |a| = r the FP32 |a|^2 is within D E (2.02 r + 2E) +
+    // 3 2^-24 |a|^2 of the FP64 value; the band is twice that, and the
+    // thresholds are rounded outwards.
     for (int a = 0; a < 3; ++a) {
         g.L32[a] = (float)g.L[a];
         g.invL32[a] = (float)g.invL[a];
@@ -368,18 +368,23 @@ Geom make_geom(const pgt_ik_engine *e, int D)
         Lm = std::max(Lm, e->L[a]);
         Lmin = std::min(Lmin, e->L[a]);
     }
-    const double rc = e->prm.cg_rcut;
-    const double Ee = (4 * M + 2 * Lm + 4 * rc) * std::ldexp(1.0, -23);
-    const double B = 2 * (D * Ee * (2.02 * rc + 2 * Ee) + 3 * std::ldexp(1.0, -24) * 1.03 * g.rc2);
     // (a search radius well below half of every box length, so that the FP32
     // and FP64 minimum images can differ only for candidates far outside)
-    if (ok && g.rc2 - B > 0 && 2 * rc + 8 * Ee < Lmin) {
-        float lo32 = (float)(g.rc2 - B), hi32 = (float)(g.rc2 + B);
-        if ((double)lo32 > g.rc2 - B) lo32 = std::nextafter(lo32, -INFINITY);
-        if ((double)hi32 < g.rc2 + B) hi32 = std::nextafter(hi32, INFINITY);
-        g.in32 = lo32;
-        g.out32 = hi32;
-    }
+    auto band = [&](double r, float &in32, float &out32) {
+        in32 = -1.0f;
+        out32 = INFINITY;
+        const double r2 = r * r;
+        const double Ee = (4 * M + 2 * Lm + 4 * r) * std::ldexp(1.0, -23);
+        const double B = 2 * (D * Ee * (2.02 * r + 2 * Ee) + 3 * std::ldexp(1.0, -24) * 1.03 * r2);
+        if (!(ok && r2 - B > 0 && 2 * r + 8 * Ee < Lmin)) return;
+        float lo32 = (float)(r2 - B), hi32 = (float)(r2 + B);
+        if ((double)lo32 > r2 - B) lo32 = std::nextafter(lo32, -INFINITY);
+        if ((double)hi32 < r2 + B) hi32 = std::nextafter(hi32, INFINITY);
+        in32 = lo32;
+        out32 = hi32;
+    };
+    band(e->prm.cg_rcut, g.in32, g.out32);
+    band(e->prm.max_rcut, g.bin32, g.bout32);
     return g;
 }
 
@@ -552,6 +557,7 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     if (kin && (rc = need(e->d_skin, (size_t)nf * n * KN))) return rc;
     if (ela && (rc = need(e->d_sdisp, (size_t)nf * n * 3))) return rc;
     if (pairs && D == 3 && (rc = need(e->d_rec, (size_t)nf * n))) return rc;
+    float4 *rec = pairs && D == 3 ? e->d_rec.p : nullptr;
     hipStream_t s = e->stream;
     if (ev_begin) HIPCHK(hipEventRecord(ev_begin, s));
     double *skin = kin ? e->d_skin.p : nullptr;
@@ -582,13 +588,13 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
         const dim3 pgrid((unsigned)((n + 255) / 256), (unsigned)frame_rows(nf));
         if (D == 2)
             hipLaunchKernelGGL((k_sort_finish<2, 2>), pgrid, dim3(256), 0, s, g, n, nf, g.ncell, keys, vals, pos, diam,
-                               mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
+                               mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp, rec);
         else if (OD == 2)
             hipLaunchKernelGGL((k_sort_finish<3, 2>), pgrid, dim3(256), 0, s, g, n, nf, g.ncell, keys, vals, pos, diam,
-                               mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
+                               mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp, rec);
         else
             hipLaunchKernelGGL((k_sort_finish<3, 3>), pgrid, dim3(256), 0, s, g, n, nf, g.ncell, keys, vals, pos, diam,
-                               mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp);
+                               mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp, rec);
     }
     HIPCHK(hipGetLastError());
 
@@ -647,7 +653,7 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
         c.cap = 1;
     }
     c.bcount = e->d_bcount.p;
-    c.rec = pairs && D == 3 ? e->d_rec.p : nullptr;
+    c.rec = rec;
     c.bonds = e->d_bonds.p;
     c.pack = n < (1 << (c.pl > 1 ? 22 : 26)) && c.cap <= 63;
     if (ev_gather) HIPCHK(hipEventRecord(ev_gather, s));

@@ -28,12 +28,14 @@ struct Geom {
     double inv_rc;   // 1/cg_rcut
     double inv_rc2;  // 1/cg_rcut^2
     double mr2;      // max_rcut^2
-    // FP32 pre-test of the gather's candidate scan: with |a|^2 from FP32
-    // copies of the positions, |a|^2 < in32 means inside the ball and
-    // |a|^2 > out32 outside, whatever the FP64 rounding; in between the FP64
-    // test decides.  in32 < 0 and out32 = inf send every candidate to it.
+    // FP32 pre-tests of the 3D candidate scans: with |a|^2 from FP32 copies
+    // of the positions, |a|^2 < in32 means inside the coarse-graining ball
+    // and |a|^2 > out32 outside, whatever the FP64 rounding; in between the
+    // FP64 test decides (in32 < 0 and out32 = inf send every candidate to
+    // it).  bin32 and bout32 do the same for the bond search radius.
     float L32[3], invL32[3];
     float in32, out32;
+    float bin32, bout32;
 };
 
 // Minimum image with the image index rint(d/L) taken as rint(d * (1/L)).
@@ -297,7 +299,7 @@ __global__ __launch_bounds__(256) void k_sort_finish(Geom g, int n, int nf, int 
                                                      const double *__restrict__ vel, const double *__restrict__ disp,
                                                      int *__restrict__ cstart, double *__restrict__ spos,
                                                      double *__restrict__ sdiam, double *__restrict__ skin,
-                                                     double *__restrict__ sdisp)
+                                                     double *__restrict__ sdisp, float4 *__restrict__ rec)
 {
     constexpr int PS = (D == 2) ? 2 : 4;
     constexpr int KN = OD * (OD + 1) / 2;
@@ -321,6 +323,9 @@ __global__ __launch_bounds__(256) void k_sort_finish(Geom g, int n, int nf, int 
     sp[0] = wrap(0);
     sp[1] = wrap(1);
     if (D == 3) { sp[2] = wrap(2); sp[3] = 0.0; }
+    // 3D with bonds: the FP32 record of the candidate scans (its last word,
+    // the bond count, is filled in by the bond kernel)
+    if (D == 3 && rec) rec[fk] = make_float4((float)sp[0], (float)sp[1], (float)sp[2], 0.0f);
     sdiam[fk] = diam[fi];
     if (skin) {
         const double m = mass ? mass[fi] : 1.0;

@@ -228,6 +228,35 @@ def test_3d_candidates_on_the_ball_surface(gpu, precision):
                 assert np.abs(got["virial"][:, b, a] - r).max() <= 1e-5 * np.abs(r).max()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_3d_pairs_on_the_search_radius(gpu, precision):
+    """The 3D bond list tests pair distances in FP32 first and leaves those
+    near max_rcut to the FP64 test.  With max_rcut = 1.0, below the cutoff
+    of 108 of the 120 pairs placed at 1.0 (1 +- 1e-13 ... 1e-5), a pair
+    listed wrongly would change the contribution count and the field."""
+    sysd = workload.make_system(3000, 3, seed=72)
+    rng = np.random.default_rng(9)
+    idx = rng.choice(len(sysd["diam"]), 240, replace=False)
+    u = rng.normal(size=(120, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    rel = np.array([-1e-5, -1e-7, -1e-10, -1e-13, 1e-13, 1e-10, 1e-7, 1e-5])
+    L = sysd["L"]
+    pos = sysd["pos"].copy()
+    pos[idx[1::2]] = pos[idx[0::2]] + 1.0 * u * (1.0 + rel[np.arange(120) % len(rel)])[:, None]
+    pos = pos - L * np.floor((pos + L / 2) / L)
+    sysd = dict(sysd, pos=pos)
+    grid = workload.cartesian_grid_3d(L[0], 8)
+    got, ref, st = run_both(sysd, grid, fields=("virial",), precision=precision, max_rcut=1.0)
+    if precision == "fp64":
+        assert_parity(got, ref, st, ("virial",))
+    else:
+        assert st["n_ball"] == ref["n_ball"]
+        for a in range(3):
+            for b in range(a, 3):
+                r = ref["virial"][:, b, a]
+                assert np.abs(got["virial"][:, b, a] - r).max() <= 1e-5 * np.abs(r).max()
+
+
 def test_2d_system_with_3d_fields_nonflat_z(gpu):
     """A 2D box whose particles sit off z=0 takes the general 3-component path."""
     sysd = workload.make_system(1500, 2, seed=5)
