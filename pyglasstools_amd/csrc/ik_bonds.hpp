@@ -50,7 +50,10 @@ struct BondLayout {
 // with the lanes of a wavefront in step instead of behind a branch that some
 // lane takes at almost every candidate.  A full list is processed early.
 // Entries keep the candidate order (cells in a fixed order, input order
-// inside a cell).
+// inside a cell).  In 3D pass 1 reads the FP32 records and tests distances in
+// FP32 against bounds of the rounding (Geom::bin32, bout32), falling back to
+// the FP64 test in the band between them, so the list is that of the FP64
+// test; the kernel then stores each particle's bond count in its record.
 template <int D, int OD, bool VIR, bool ELA, typename ER, int BLK, int MAXC>
 __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, double cgnorm,
                                                const int *__restrict__ cstart, const double *__restrict__ spos,

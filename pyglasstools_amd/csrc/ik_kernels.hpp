@@ -291,7 +291,9 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(int n, int nf, int 
 // cstart[ncell] = n (keys are sorted, so slot k writes the starts of the
 // cells after its predecessor's up to its own).  Positions are wrapped into
 // the box along periodic axes (x - L floor((x - lo) / L): unchanged, bit for
-// bit, inside the box), so a particle sits where its cell says.
+// bit, inside the box), so a particle sits where its cell says.  In 3D with
+// bonds it also writes each particle's FP32 record (x, y, z; the bond kernel
+// adds the count) for the candidate scans' FP32 pre-tests.
 template <int D, int OD>
 __global__ __launch_bounds__(256) void k_sort_finish(Geom g, int n, int nf, int ncell, const int *__restrict__ keys,
                                                      const int *__restrict__ vals, const double *__restrict__ pos,
