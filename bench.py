@@ -395,7 +395,7 @@ def batched_sweep(args):
             nc_graph += j["e"].stats()["n_contrib"]
         # (c) one graph whose systems run on K streams forked from the capture
         # stream and joined back (the largest system paired with the smallest)
-        K = 4
+        K = args.sweep_streams
         side = [torch.cuda.Stream() for _ in range(K)]
         order = sorted(range(len(jobs)), key=lambda i: jobs[i]["sy"]["N"])
         lane = {}
@@ -437,15 +437,15 @@ def batched_sweep(args):
             (prec, nframes / t_graph, K, nframes / t_graph_k, nframes / t_eager))
         del jobs, g, g2
         torch.cuda.empty_cache()
-    best = results["fp64"]["frames_per_s_graph_4_streams"]
+    best = results["fp64"]["frames_per_s_graph_%d_streams" % args.sweep_streams]
     line = {"metric": METRIC, "value": round(best, 3), "unit": "frames/s",
             "n_gpus": 1, "steps": args.steps, "warmup": 1,
             "ms_per_step": round(FR * len(C5_N) / best * 1e3, 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (seeded lattice+jitter polydisperse glasses, 8 systems x 125 frames resident in HBM)",
             "config": {"workload": "2D polydisperse12, 8 systems N in %s x 125 frames, 128x128 grid each, Octic "
-                                   "r_cg=2.0, GL5, virial + kinetic; one HIP graph per sweep, systems on 4 streams forked "
-                                   "inside it" % list(C5_N),
+                                   "r_cg=2.0, GL5, virial + kinetic; one HIP graph per sweep, systems on %d streams forked "
+                                   "inside it" % (list(C5_N), args.sweep_streams),
                        "frames_per_step": FR * len(C5_N), "grid": "128x128", "parallelism": "single GPU"},
             "roofline": None, "cpu_baseline": None,
             "extra": {"fp64": results["fp64"], "fp32_form": results["fp32"],
@@ -461,6 +461,8 @@ def main():
                          "5: batched sweep over 8 systems of uneven N in one HIP graph")
     ap.add_argument("--c3-sample", type=int, default=2048, help="grid points of the config-3 oracle check")
     ap.add_argument("--c4-epochs", type=int, default=60, help="training epochs of the config-4 surrogate")
+    ap.add_argument("--sweep-streams", type=int, default=4, choices=[1, 2, 4, 8],
+                    help="streams the config-5 graph forks its systems onto")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
