@@ -18,7 +18,7 @@ INCLUDE = os.path.join(ROOT, "include")
 ARCH = os.environ.get("PGT_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-O3", "-std=c++17", "-fPIC",
     "--offload-arch=" + ARCH,
     # keep FP32 arithmetic off the packed-FP32 VALU instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -39,10 +39,30 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
+# translation units of libpgt_ik.so, compiled in parallel: the host engine
+# with the cell-list, bond-list and surrogate kernels, and the field gathers
+ENGINE_SOURCES = ["ik_gather_3d64.hip", "ik_gather_3d32.hip", "ik_gather_3d2.hip", "ik_gather_2d64.hip",
+                  "ik_gather_2d32.hip", "ik_engine.hip", "ik_gather_adaptive.hip"]
+
+
 def build_engine():
+    import shutil
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
+
     out = lib_path()
     tmp = out + ".tmp"
-    _run(["hipcc"] + HIP_FLAGS + ["-I" + INCLUDE, "-o", tmp, os.path.join(HERE, "ik_engine.hip")])
+    objdir = tempfile.mkdtemp(prefix="pgt_build_")
+    try:
+        objs = [os.path.join(objdir, os.path.splitext(s)[0] + ".o") for s in ENGINE_SOURCES]
+        jobs = [["hipcc"] + HIP_FLAGS + ["-I" + INCLUDE, "-c", "-o", o, os.path.join(HERE, s)]
+                for s, o in zip(ENGINE_SOURCES, objs)]
+        workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS") or 0) or os.cpu_count() or 1))
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            list(ex.map(_run, jobs))
+        _run(["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
+    finally:
+        shutil.rmtree(objdir, ignore_errors=True)
     os.replace(tmp, out)
     return out
 

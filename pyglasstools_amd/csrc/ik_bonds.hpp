@@ -6,6 +6,7 @@
 #include <type_traits>
 
 #include "ik_device.hpp"
+#include "ik_geom.hpp"
 
 namespace pgt {
 

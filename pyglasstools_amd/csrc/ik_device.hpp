@@ -136,6 +136,29 @@ struct GLRule {
     double w[kMaxQArg];
 };
 
+// sum_k w_k shape(t(s_k)), t(s) = A + s B + s^2 C: the bond integral of the
+// unnormalised kernel along the segment, in units of the kernel width.
+template <int CG, int NQ, typename ER>
+__device__ __forceinline__ ER bond_quad(const GLRule &R, const double *__restrict__ gld, int nq, ER A, ER B, ER C)
+{
+    ER acc = ER(0);
+    if constexpr (NQ > 0) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            const ER s = (ER)R.s[k];
+            const ER t = fma(s, fma(s, C, B), A);
+            acc = fma((ER)R.w[k], shape<CG, ER>(t), acc);
+        }
+    } else {
+        for (int k = 0; k < nq; ++k) {
+            const ER s = (ER)gld[k];
+            const ER t = fma(s, fma(s, C, B), A);
+            acc = fma((ER)gld[nq + k], shape<CG, ER>(t), acc);
+        }
+    }
+    return acc;
+}
+
 // Adaptive quadrature (AdaptiveCGFunc::getBondFunc, CoarseGrainFunction.h:
 // 124-127 -> gsl_quad::integrate, Quadrature.h:39-51: gsl_integration_qag
 // with key 1, the 15-point Gauss-Kronrod rule).  The rule's abscissae and

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "ik_kernels.hpp"
+#include "ik_launch.hpp"
 #include "pgt_ik.h"
 
 using namespace pgt;
@@ -391,95 +392,6 @@ Geom make_geom(const pgt_ik_engine *e, int D)
 // ------------------------------------------------------------------------
 // launch dispatch
 // ------------------------------------------------------------------------
-struct LaunchCtx {
-    Geom g;
-    int n, G, nf;
-    int cap;
-    int block;
-    int pl;     // grid points per gather lane
-    bool pack;  // LDS list entries packed into one int (see k_gather)
-    mutable int pl_launched;  // points per lane of the gather actually launched
-    const double *grid;
-    const int *perm;
-    GLRule R;
-    const double *gld;
-    double cgnorm;
-    const int *cstart;
-    const double *spos, *skin;
-    const int *bcount;
-    const float4 *rec;  // FP32 position and bond count per sorted particle
-    const void *bonds;
-    GatherOut out;
-    hipStream_t stream;
-};
-
-// Gather points per lane in use: the tuned value where launch_gather_c
-// picks the unrolled Octic 5-node rule, 1 elsewhere.
-inline int gather_pl(int tuned, int cg, int nq, int D, int OD)
-{
-    return (cg == 0 && nq == 5 && !(D == 3 && OD == 2)) ? tuned : 1;
-}
-
-template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, typename ER, int BLK, int PL>
-void launch_gather_t(const LaunchCtx &c)
-{
-    // per-lane list length: ~2x the mean number of particles in a ball at
-    // unit density (12.6 in 2D, 33.5 in 3D for r_cg = 2), more for the union
-    // of several points' balls; longer lists are processed in pieces
-    constexpr int MAXL = (D == 2 ? 24 : 48) * (PL == 1 ? 3 : PL == 2 ? 4 : 5) / 3;
-    dim3 grid((unsigned)((c.G + BLK * PL - 1) / (BLK * PL)), (unsigned)c.nf);
-    c.pl_launched = PL;
-    if (c.pack)
-        hipLaunchKernelGGL((k_gather<D, OD, CG, NQ, VIR, ELA, ER, BLK, MAXL, true, PL>), grid, dim3(BLK), 0, c.stream,
-                           c.g, c.n, c.G, c.grid, c.perm, c.R, c.gld, c.cgnorm, c.cstart, c.spos, c.skin, c.cap,
-                           c.bcount, c.rec, (const ER *)c.bonds, c.out);
-    else
-        hipLaunchKernelGGL((k_gather<D, OD, CG, NQ, VIR, ELA, ER, BLK, MAXL, false, PL>), grid, dim3(BLK), 0,
-                           c.stream, c.g, c.n, c.G, c.grid, c.perm, c.R, c.gld, c.cgnorm, c.cstart, c.spos, c.skin,
-                           c.cap, c.bcount, c.rec, (const ER *)c.bonds, c.out);
-}
-
-template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, typename ER, int BLK>
-void launch_gather_p(const LaunchCtx &c)
-{
-    if constexpr (CG == 0 && NQ == 5) {
-        if (c.pl == 4) return launch_gather_t<D, OD, CG, NQ, VIR, ELA, ER, BLK, 4>(c);
-        if (c.pl == 2) return launch_gather_t<D, OD, CG, NQ, VIR, ELA, ER, BLK, 2>(c);
-    }
-    launch_gather_t<D, OD, CG, NQ, VIR, ELA, ER, BLK, 1>(c);
-}
-
-template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, typename ER>
-void launch_gather_b(const LaunchCtx &c)
-{
-    if (c.block == 128) launch_gather_p<D, OD, CG, NQ, VIR, ELA, ER, 128>(c);
-    else launch_gather_p<D, OD, CG, NQ, VIR, ELA, ER, 256>(c);
-}
-
-template <int D, int OD, int CG, int NQ, typename ER>
-void launch_gather_f(const LaunchCtx &c, unsigned fields)
-{
-    const bool V = fields & PGT_FIELD_VIRIAL, E = fields & PGT_FIELD_ELASTIC;
-    if (V && E) launch_gather_b<D, OD, CG, NQ, true, true, ER>(c);
-    else if (V) launch_gather_b<D, OD, CG, NQ, true, false, ER>(c);
-    else if (E) launch_gather_b<D, OD, CG, NQ, false, true, ER>(c);
-    else launch_gather_b<D, OD, CG, NQ, false, false, ER>(c);
-}
-
-// The Octic kernel gets the unrolled 5-node rule (the driver scripts' order)
-// and the FP32 form; Rect and Mollifier run the generic FP64 rule.
-template <int D, int OD, typename ER>
-void launch_gather_c(const LaunchCtx &c, unsigned fields, int cg)
-{
-    if (cg == 0) {
-        if (c.R.n == 5 && !(D == 3 && OD == 2)) launch_gather_f<D, OD, 0, 5, ER>(c, fields);
-        else launch_gather_f<D, OD, 0, 0, ER>(c, fields);
-    } else if constexpr (std::is_same<ER, double>::value) {
-        if (cg == 1) launch_gather_f<D, OD, 1, 0, double>(c, fields);
-        else launch_gather_f<D, OD, 2, 0, double>(c, fields);
-    }
-}
-
 constexpr int kBondBlock = 128;  // k_bonds threads per workgroup
 
 template <int D, int OD, typename ER>
@@ -662,31 +574,15 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
         // cell list and bond list only (the surrogate field form reads them)
     } else if (adaptive) {
         const AdaptQ Q{e->prm.adapt_limit, e->prm.adapt_epsabs, e->prm.adapt_epsrel};
-        const dim3 ag((unsigned)((c.G + 127) / 128), (unsigned)c.nf);
-        const bool V = fields & PGT_FIELD_VIRIAL, E = fields & PGT_FIELD_ELASTIC;
-#define PGT_LAUNCH_ADAPT(DD, OO, VV, EE)                                                                          \
-    hipLaunchKernelGGL((k_gather_adaptive<DD, OO, VV, EE>), ag, dim3(128), 0, s, c.g, c.n, c.G, c.grid, c.perm, cg, \
-                       Q, c.cgnorm, c.cstart, c.spos, c.skin, c.cap, c.bcount, (const double *)c.bonds, c.out,       \
-                       e->d_maxcount.p + 1)
-#define PGT_LAUNCH_ADAPT_F(DD, OO)                              \
-    do {                                                        \
-        if (V && E) PGT_LAUNCH_ADAPT(DD, OO, true, true);       \
-        else if (E) PGT_LAUNCH_ADAPT(DD, OO, false, true);      \
-        else PGT_LAUNCH_ADAPT(DD, OO, true, false);             \
-    } while (0)
-        if (D == 2) PGT_LAUNCH_ADAPT_F(2, 2);
-        else if (OD == 2) PGT_LAUNCH_ADAPT_F(3, 2);
-        else PGT_LAUNCH_ADAPT_F(3, 3);
-#undef PGT_LAUNCH_ADAPT_F
-#undef PGT_LAUNCH_ADAPT
+        launch_gather_adaptive(c, fields, cg, D, OD, Q, e->d_maxcount.p + 1);
     } else if (D == 2) {
-        if (fp32) launch_gather_c<2, 2, float>(c, fields, cg);
-        else launch_gather_c<2, 2, double>(c, fields, cg);
+        if (fp32) launch_gather_2d32(c, fields, cg);
+        else launch_gather_2d64(c, fields, cg);
     } else if (OD == 2) {
-        launch_gather_c<3, 2, double>(c, fields, cg);
+        launch_gather_3d2(c, fields, cg);
     } else {
-        if (fp32) launch_gather_c<3, 3, float>(c, fields, cg);
-        else launch_gather_c<3, 3, double>(c, fields, cg);
+        if (fp32) launch_gather_3d32(c, fields, cg);
+        else launch_gather_3d64(c, fields, cg);
     }
     HIPCHK(hipGetLastError());
     if (!bonds_only && !adaptive && c.pl_launched != c.pl)

@@ -1,47 +1,15 @@
 // ik_gather.hpp -- the field-gather kernel of the IK stress-field engine.
-// Included by ik_kernels.hpp.
+// Included by the gather translation units (ik_gather_*.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
 
 #include "ik_device.hpp"
+#include "ik_bonds.hpp"
+#include "ik_geom.hpp"
 
 namespace pgt {
-
-struct GatherOut {
-    double *vir;
-    double *kin;
-    double *ela;
-    // per-wavefront counts (balls, contributions), two per wavefront in
-    // launch order; summed by k_reduce_stats.  One device-scope atomic per
-    // wavefront on a shared counter would serialise every wavefront of the
-    // launch on a single address.
-    unsigned long long *wave_counts;
-};
-
-// sum_k w_k shape(t(s_k)), t(s) = A + s B + s^2 C: the bond integral of the
-// unnormalised kernel along the segment, in units of the kernel width.
-template <int CG, int NQ, typename ER>
-__device__ __forceinline__ ER bond_quad(const GLRule &R, const double *__restrict__ gld, int nq, ER A, ER B, ER C)
-{
-    ER acc = ER(0);
-    if constexpr (NQ > 0) {
-#pragma unroll
-        for (int k = 0; k < NQ; ++k) {
-            const ER s = (ER)R.s[k];
-            const ER t = fma(s, fma(s, C, B), A);
-            acc = fma((ER)R.w[k], shape<CG, ER>(t), acc);
-        }
-    } else {
-        for (int k = 0; k < nq; ++k) {
-            const ER s = (ER)gld[k];
-            const ER t = fma(s, fma(s, C, B), A);
-            acc = fma((ER)gld[nq + k], shape<CG, ER>(t), acc);
-        }
-    }
-    return acc;
-}
 
 // N values (a whole number of 16-byte pieces, 16-byte aligned) in 16-byte loads.
 template <typename ER, int ES>

@@ -1,0 +1,33 @@
+// ik_gather_adaptive.hip -- instantiations of the field gather with the
+// adaptive (QAG) bond integral.
+#include "ik_gather_adaptive.hpp"
+#include "ik_launch.hpp"
+#include "pgt_ik.h"
+
+namespace pgt {
+
+template <int D, int OD>
+static void launch_adaptive_d(const LaunchCtx &c, unsigned fields, int cg, AdaptQ Q, int *capped)
+{
+    const dim3 grid((unsigned)((c.G + 127) / 128), (unsigned)c.nf);
+    const bool V = fields & PGT_FIELD_VIRIAL, E = fields & PGT_FIELD_ELASTIC;
+    const double *bonds = (const double *)c.bonds;
+    if (V && E)
+        hipLaunchKernelGGL((k_gather_adaptive<D, OD, true, true>), grid, dim3(128), 0, c.stream, c.g, c.n, c.G, c.grid,
+                           c.perm, cg, Q, c.cgnorm, c.cstart, c.spos, c.skin, c.cap, c.bcount, bonds, c.out, capped);
+    else if (E)
+        hipLaunchKernelGGL((k_gather_adaptive<D, OD, false, true>), grid, dim3(128), 0, c.stream, c.g, c.n, c.G, c.grid,
+                           c.perm, cg, Q, c.cgnorm, c.cstart, c.spos, c.skin, c.cap, c.bcount, bonds, c.out, capped);
+    else
+        hipLaunchKernelGGL((k_gather_adaptive<D, OD, true, false>), grid, dim3(128), 0, c.stream, c.g, c.n, c.G, c.grid,
+                           c.perm, cg, Q, c.cgnorm, c.cstart, c.spos, c.skin, c.cap, c.bcount, bonds, c.out, capped);
+}
+
+void launch_gather_adaptive(const LaunchCtx &c, unsigned fields, int cg, int D, int OD, AdaptQ Q, int *capped)
+{
+    if (D == 2) launch_adaptive_d<2, 2>(c, fields, cg, Q, capped);
+    else if (OD == 2) launch_adaptive_d<3, 2>(c, fields, cg, Q, capped);
+    else launch_adaptive_d<3, 3>(c, fields, cg, Q, capped);
+}
+
+}  // namespace pgt

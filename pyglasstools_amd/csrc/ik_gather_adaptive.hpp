@@ -1,9 +1,11 @@
 // ik_gather_adaptive.hpp -- field gather with the adaptive bond integral.
-// Included by ik_kernels.hpp.
+// Included by ik_gather_adaptive.hip.
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "ik_bonds.hpp"
 #include "ik_device.hpp"
+#include "ik_geom.hpp"
 
 namespace pgt {
 

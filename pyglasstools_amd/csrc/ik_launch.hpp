@@ -1,0 +1,55 @@
+// ik_launch.hpp -- launch context shared by the engine's host code and the
+// field-gather translation units (ik_gather_*.hip), which are compiled separately and in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ik_device.hpp"
+#include "ik_geom.hpp"
+
+namespace pgt {
+
+// Everything a batch's bond-list and gather launches read.
+struct LaunchCtx {
+    Geom g;
+    int n, G, nf;
+    int cap;
+    int block;
+    int pl;     // grid points per gather lane
+    bool pack;  // LDS list entries packed into one int (see k_gather)
+    mutable int pl_launched;  // points per lane of the gather actually launched
+    const double *grid;
+    const int *perm;
+    GLRule R;
+    const double *gld;
+    double cgnorm;
+    const int *cstart;
+    const double *spos, *skin;
+    const int *bcount;
+    const float4 *rec;  // FP32 position and bond count per sorted particle
+    const void *bonds;
+    GatherOut out;
+    hipStream_t stream;
+};
+
+// Gather points per lane in use: the tuned value where the gather launchers
+// pick the unrolled Octic 5-node rule, 1 elsewhere.
+inline int gather_pl(int tuned, int cg, int nq, int D, int OD)
+{
+    return (cg == 0 && nq == 5 && !(D == 3 && OD == 2)) ? tuned : 1;
+}
+
+// Field gather of one batch (fixed-order rule), one function per form and
+// translation unit: 2D positions with FP64 or FP32 bond entries, 3D positions
+// with 2 x 2 output tensors (a 2D system off the z = 0 plane), and 3D with
+// FP64 or FP32 entries.  fields: PGT_FIELD_* bits; cg: kernel kind.
+void launch_gather_2d64(const LaunchCtx &c, unsigned fields, int cg);  // ik_gather_2d64.hip
+void launch_gather_2d32(const LaunchCtx &c, unsigned fields, int cg);  // ik_gather_2d32.hip
+void launch_gather_3d2(const LaunchCtx &c, unsigned fields, int cg);   // ik_gather_3d2.hip
+void launch_gather_3d64(const LaunchCtx &c, unsigned fields, int cg);  // ik_gather_3d64.hip
+void launch_gather_3d32(const LaunchCtx &c, unsigned fields, int cg);  // ik_gather_3d32.hip
+// Field gather with the adaptive (QAG) bond integral; capped gets 1 when an
+// integral needs more intervals than the workspace holds.
+void launch_gather_adaptive(const LaunchCtx &c, unsigned fields, int cg, int D, int OD, AdaptQ Q,
+                            int *capped);  // ik_gather_adaptive.hip
+
+}  // namespace pgt
