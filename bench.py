@@ -43,10 +43,12 @@ def make_frames(nframes, n, config=2):
     return systems
 
 
-def traffic_from_profile():
-    """HBM bytes per gather launch measured by rocprofv3 PMC counters, if a
-    committed summary exists (profiles/gather_c2_traffic.json)."""
-    p = os.path.join(ROOT, "profiles", "gather_c2_traffic.json")
+def traffic_from_profile(mode):
+    """HBM bytes per launch of the C2 field gather of the given kind ("tile"
+    or "lane") measured by rocprofv3 PMC counters (FETCH_SIZE and WRITE_SIZE,
+    calibrated on a device copy), if a committed summary exists
+    (profiles/gather_c2_<mode>_traffic.json)."""
+    p = os.path.join(ROOT, "profiles", "gather_c2_%s_traffic.json" % mode)
     if os.path.exists(p):
         try:
             with open(p) as f:
@@ -57,12 +59,44 @@ def traffic_from_profile():
     return None, None
 
 
+def host_cpu():
+    """CPU model, logical CPUs online, and the CPUs this process may use (its
+    affinity mask: the host's share given to this job)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "cpus_online": os.cpu_count(), "cpus_usable": usable}
+
+
 def cpu_baseline(systems, grid, max_rcut, seconds=10.0):
-    """Oracle on the host: frames in order until `seconds` of wall time."""
+    """Oracle on the host: frames in order until `seconds` of wall time, on
+    every CPU this process may use (the affinity mask; an OMP_NUM_THREADS set
+    by the host environment caps it), built for this host's CPU
+    (-march=native, as the reference's CompilerFlagsSetup.cmake:14 builds)."""
+    import tempfile
+
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
-    threads = max(1, min(threads, 16))
+    info = host_cpu()
+    threads = info["cpus_usable"]
+    env_threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if env_threads > 0:
+        threads = min(threads, env_threads)
+    build = "-O3 -march=native -fopenmp (built on this host)"
+    try:
+        oracle.use_library(oracle.build_native(tempfile.mkdtemp(prefix="pgt_oracle_")))
+    except Exception as exc:  # no compiler on this host: the portable build
+        build = "prebuilt -O3 -march=x86-64-v3 -fopenmp (native build failed: %s)" % exc
     out = {}
     for mode, name in ((1, "clean"), (0, "faithful")):
         done, t0 = 0, time.perf_counter()
@@ -74,7 +108,72 @@ def cpu_baseline(systems, grid, max_rcut, seconds=10.0):
                 break
         dt = time.perf_counter() - t0
         out[name] = {"frames": done, "seconds": dt, "fps": done / dt}
-    return out, threads
+    info["build"] = build
+    info["threads"] = threads
+    return out, threads, info
+
+
+DRIVER_SCRIPT = """
+import json, sys, time
+t_start = time.perf_counter()
+sys.path.insert(0, {root!r})
+import pyglasstools as pglass
+import pyglasstools.io as io
+import pyglasstools.utils as utils
+import pyglasstools.potential as pot
+import pyglasstools.irvingkirkwood as ik
+import pyglasstools.irvingkirkwood.cgfunc as cgfunc
+poly12 = pot.polydisperse12(v0=1.0, eps=0.2, rcut=1.25)
+mycgfunc = cgfunc.octic(rcut=2.0, mode="fixedpoint", order=5)
+mysystem = utils.read_gsd(filename={gsd!r}, checkpointfile="restart.log")
+MyCalculator = ik.ikcalculator(cgfunc=mycgfunc, dx=0.25)
+names = ['virialstress_00', 'virialstress_11', 'virialstress_01']
+mylogger = io.fieldlogger(keyword="test-dump", names=names, solver=MyCalculator)
+t_ready = time.perf_counter()
+acc = {{"update": 0.0, "run": 0.0, "save": 0.0}}
+def timed(obj, name, key):
+    f = getattr(obj, name)
+    def g(*a, **k):
+        t = time.perf_counter()
+        r = f(*a, **k)
+        acc[key] += time.perf_counter() - t
+        return r
+    setattr(obj, name, g)
+timed(MyCalculator, "update", "update")
+timed(MyCalculator, "run", "run")
+timed(mylogger, "save", "save")
+t0 = time.perf_counter()
+pglass.analyze(frame_list=range(180, 200))
+t1 = time.perf_counter()
+print("DRIVER " + json.dumps({{"frames": 20, "analyze_s": t1 - t0, "setup_s": t_ready - t_start,
+                              "script_s": t1 - t_start, "update_s": acc["update"], "compute_s": acc["run"],
+                              "save_s": acc["save"]}}))
+"""
+
+
+def driver_script_time():
+    """The reference's IK example sequence (example_scripts/irvingkirkwood/
+    test-irvingkirkwood-module.py: potential, kernel, GSD ingest,
+    ikcalculator, fieldlogger, analyze) on the committed fixture, frames
+    180..199 (1024 particles, 127 x 127 grid), run as a script through the
+    pybind11 surface: wall time of the whole script and of its parts."""
+    import subprocess
+    import tempfile
+
+    gsd = os.path.join(ROOT, "tests", "golden", "polydisperse12.gsd")
+    with tempfile.TemporaryDirectory(prefix="pgt_driver_") as d:
+        with open(os.path.join(d, "drive.py"), "w") as f:
+            f.write(DRIVER_SCRIPT.format(root=ROOT, gsd=gsd))
+        t0 = time.perf_counter()
+        r = subprocess.run([sys.executable, "drive.py"], cwd=d, capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        return {"error": r.stderr[-500:]}
+    line = [x for x in r.stdout.splitlines() if x.startswith("DRIVER ")][-1]
+    out = json.loads(line[len("DRIVER "):])
+    out["process_wall_s"] = wall
+    out["compute_ms_per_frame"] = out["compute_s"] / out["frames"] * 1e3
+    return out
 
 
 def parity_sample(systems, grid, max_rcut, got_virial, got_kinetic, stride=16):
@@ -182,10 +281,13 @@ def precision_form(args):
             "config": {"workload": "3D polydisperse12 N=131072, 20 frames, 64^3 grid, Octic r_cg=2.0, GL5, FP32 "
                                    "forces and kernel weights, FP64 minimum image and accumulation, virial stress",
                        "frames_per_step": F, "n_particles": N, "grid": "64^3", "parallelism": "single GPU"},
-            "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+            "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
                          "kernel": "k_gather (field gather, FP32 form)",
-                         "compute": "FP32 VALU (published MI355X FP32 peak 157.3 TFLOP/s)",
+                         "compute": "FP32 vector ALU (published MI355X FP32 peak 157.3 TFLOP/s, which counts packed "
+                                    "two-wide FP32; this build issues unpacked FP32, whose rate equals the FP64 "
+                                    "rate of 78.6 TFLOP/s)",
+                         "frac_of_unpacked_rate": round(achieved / PEAK_FP64_TFLOPS, 4),
                          "avg_launch_ms": round(avg_launch_s * 1e3, 4), "flops_per_launch": flops},
             "cpu_baseline": None,
             "extra": {"contributions_per_s": n_contrib / elapsed, "contributions_per_frame": n_contrib / (F * args.steps),
@@ -477,6 +579,8 @@ def main():
     ap.add_argument("--fpl", type=int, default=0, help="frames per launch (0: autotune)")
     ap.add_argument("--pl", type=int, default=0, help="grid points per gather lane (0: autotune)")
     ap.add_argument("--reach", type=int, default=1, help="cell reach with --block/--fpl/--pl fixed (1 or 2)")
+    ap.add_argument("--gather", default="tile", choices=["tile", "lane"],
+                    help="field gather for the Octic 5-node rule: LDS-staged tiles or one lane per point")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (reference dispatches for HBM byte counters) and exit")
     args = ap.parse_args()
@@ -536,6 +640,7 @@ def main():
     eng = engine.Engine(dim=2, max_rcut=max_rcut, precision=args.precision)
     eng.set_box(L, [1, 1, 0])
     eng.set_grid(grid)
+    eng.set_gather_mode(args.gather)
 
     def step():
         eng.compute_frames(F, N, pos, diam, mass, vel, None, fields=("virial", "kinetic"), d_virial=out_v,
@@ -572,13 +677,19 @@ def main():
     flops = FLOP_U_2D * n_contrib + FLOP_KIN_2D * n_ball
     avg_launch_s = gather_ms / 1e3 / launches
     achieved = flops / launches / avg_launch_s / 1e12
-    traffic, traffic_fpl = traffic_from_profile()
+    traffic, traffic_fpl = traffic_from_profile(args.gather)
     if traffic is not None and traffic_fpl != fpl:
         traffic = traffic * fpl / traffic_fpl  # per-launch bytes scale with frames per launch
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+    kname = "k_gather_tile (LDS-staged field gather)" if args.gather == "tile" else "k_gather (lane field gather)"
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
-                "kernel": "k_gather (field gather)", "compute": "FP64 VALU (published MI355X FP64 peak 78.6 TFLOP/s)",
+                "kernel": kname, "compute": "FP64 vector ALU (published MI355X FP64 peak 78.6 TFLOP/s)",
                 "avg_launch_ms": round(avg_launch_s * 1e3, 4), "flops_per_launch": flops / launches}
+    if traffic is not None:
+        # HBM bytes of the gather (PMC counters) over its launch time, against 8 TB/s
+        roofline["hbm_gbs"] = round(traffic / avg_launch_s / 1e9, 1)
+        roofline["hbm_frac"] = round(traffic / avg_launch_s / 1e9 / PEAK_HBM_GBS, 4)
+        roofline["algorithmic_bytes_per_launch"] = fpl * (N * (16 + 8 + 8 + 24) + 2 * G * 32)
 
     extra = {"frames": F, "n_particles": N, "grid_points": G, "contributions_per_s": n_contrib / elapsed,
              "contributions_per_frame": n_contrib / (F * args.steps), "kinetic_pairs_per_frame": n_ball / (F * args.steps),
@@ -603,18 +714,24 @@ def main():
         extra["host_api_fps_pcie_inclusive"] = nh / (time.perf_counter() - t0)
         host.close()
 
+    if not args.no_host:
+        extra["driver_script"] = driver_script_time()
+        log("driver script: %s" % extra["driver_script"])
+
     cpu = None
     if not args.no_cpu:
         step()
         torch.cuda.synchronize()
         extra["parity_frame0"] = parity_sample(systems, grid, max_rcut, out_v[0].cpu().numpy(), out_k[0].cpu().numpy())
         log("parity sample: %s" % extra["parity_frame0"])
-        res, threads = cpu_baseline(systems, grid, max_rcut, args.cpu_seconds)
-        log("cpu oracle: %s" % res)
+        res, threads, hinfo = cpu_baseline(systems, grid, max_rcut, args.cpu_seconds)
+        log("cpu oracle: %s on %s" % (res, hinfo))
         cpu = {"value": round(res["clean"]["fps"], 4), "unit": "frames/s", "cores": threads, "kind": "port",
                "sample": "%d of the %d frames, full 256x256 grid, FP64 oracle (C restatement of "
                          "IrvingKirkwood::compute, OpenMP over grid points, no per-triple allocations)"
-                         % (res["clean"]["frames"], F)}
+                         % (res["clean"]["frames"], F),
+               "cpu_model": hinfo["model"], "cpus_online": hinfo["cpus_online"],
+               "cpus_usable": hinfo["cpus_usable"], "build": hinfo["build"]}
         extra["cpu_faithful_fps"] = res["faithful"]["fps"]
         extra["cpu_faithful_sample_frames"] = res["faithful"]["frames"]
         extra["speedup_vs_cpu_clean"] = fps / res["clean"]["fps"]

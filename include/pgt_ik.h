@@ -161,7 +161,12 @@ int pgt_ik_compute_frames(pgt_ik_engine *e, int nframes, int64_t n, const double
  * engine's stream, see pgt_ik_set_stream) and replay them.  Needs an earlier
  * pgt_ik_compute_frames on this engine with the same frame count, particle
  * count and fields (it sizes the buffers and the bond-list capacity);
- * otherwise PGT_ERR_STATE.  Check with pgt_ik_finish.
+ * otherwise PGT_ERR_STATE.  Check with pgt_ik_finish.  From then on the
+ * engine's device buffers are held for the enqueued (or captured) launches:
+ * a later call that would free, grow or replace them (pgt_ik_set_grid, a
+ * pgt_ik_compute_frames needing more room or a larger bond capacity) fails
+ * with PGT_ERR_STATE until pgt_ik_release_enqueued; after such a call the
+ * graph must be captured again.
  * No counterpart in the reference (whose compute runs one frame at a time on
  * the CPU); it serves the batched, graph-replayed sweep over many frames and
  * systems. */
@@ -170,10 +175,16 @@ int pgt_ik_enqueue_frames(pgt_ik_engine *e, int nframes, int64_t n, const double
                           const double *d_disp, unsigned fields, double *d_out_virial,
                           double *d_out_kinetic, double *d_out_elastic);
 
+/* Declare that no captured graph will replay this engine's enqueued launches
+ * any more, so calls may again regrow its device buffers.  No counterpart in
+ * the reference. */
+int pgt_ik_release_enqueued(pgt_ik_engine *e);
+
 /* Wait for the engine's stream and check the work enqueued (or replayed)
  * since the last finish: PGT_ERR_RANGE if a particle had more bonds than the
- * capacity (those results are incomplete: run pgt_ik_compute_frames, which
- * grows it) or an adaptive integral ran out of workspace.  Sets the counts in
+ * capacity (those results are incomplete: release the enqueued work, run
+ * pgt_ik_compute_frames, which grows it, and capture again) or an adaptive
+ * integral ran out of workspace.  Sets the counts in
  * pgt_ik_get_stats.  No counterpart in the reference. */
 int pgt_ik_finish(pgt_ik_engine *e);
 
@@ -188,6 +199,15 @@ int pgt_ik_set_stream(pgt_ik_engine *e, void *hip_stream);
  * current choice.  Used by the autotuner.  No counterpart in the reference. */
 int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_launch, int points_per_lane,
                       int cell_reach);
+
+/* Field-gather kernel for the Octic 5-node rule (the driver scripts' rule):
+ * 1, the tile gather (a 128-lane workgroup stages the particles and bond
+ * entries of its window of cells in LDS once and every lane integrates from
+ * there); 0, the lane gather (each lane reads particles and entries from
+ * global memory).  Same results to rounding; other rules always use the lane
+ * gather, and a tile whose window overflows its LDS arrays makes the job run
+ * again with the lane gather.  No counterpart in the reference. */
+int pgt_ik_set_gather_mode(pgt_ik_engine *e, int mode);
 
 int pgt_ik_get_stats(pgt_ik_engine *e, pgt_ik_stats *out);
 

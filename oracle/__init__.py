@@ -85,31 +85,53 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def build_native(outdir):
+    """Compile ik_oracle.c for the host it runs on (-march=native, like the
+    reference's CompilerFlagsSetup.cmake:14) into outdir; returns the library
+    path.  Only the CPU-baseline leg of the benchmark uses this build."""
+    import subprocess
+
+    out = os.path.join(outdir, "libik_oracle_native.so")
+    subprocess.run(["gcc", "-O3", "-march=native", "-ffp-contract=off", "-fopenmp", "-fPIC", "-std=c11", "-shared",
+                    "-o", out, os.path.join(_HERE, "ik_oracle.c"), "-lm"], check=True, capture_output=True)
+    return out
+
+
+def use_library(path):
+    """Bind the oracle functions to the library at path (e.g. build_native's)."""
+    global _LIB
+    _LIB = _bind(path)
+
+
 def lib():
     global _LIB
     if _LIB is None:
         path = os.path.join(_HERE, "libik_oracle.so")
         if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(path)
-        d, i, p = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
-        L.oracle_gl_nodes.argtypes = [i, p, p]
-        L.oracle_gl_nodes.restype = i
-        L.oracle_cg_delta.argtypes = [i, d, d]
-        L.oracle_cg_delta.restype = d
-        L.oracle_bond_fixed.argtypes = [i, d, i, p, p, p]
-        L.oracle_bond_fixed.restype = d
-        L.oracle_qag.argtypes = [i, d, p, p, p, d, d, i]
-        L.oracle_qag.restype = d
-        L.oracle_force_divr.argtypes = [i, p, d, d, d, d]
-        L.oracle_force_divr.restype = d
-        L.oracle_energy.argtypes = [i, p, d, d, d, d]
-        L.oracle_energy.restype = d
-        L.oracle_ik_compute.argtypes = [ctypes.POINTER(_Args), ctypes.POINTER(_Out)]
-        L.oracle_ik_compute.restype = i
-        L.oracle_max_threads.restype = i
-        _LIB = L
+        _LIB = _bind(path)
     return _LIB
+
+
+def _bind(path):
+    L = ctypes.CDLL(path)
+    d, i, p = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+    L.oracle_gl_nodes.argtypes = [i, p, p]
+    L.oracle_gl_nodes.restype = i
+    L.oracle_cg_delta.argtypes = [i, d, d]
+    L.oracle_cg_delta.restype = d
+    L.oracle_bond_fixed.argtypes = [i, d, i, p, p, p]
+    L.oracle_bond_fixed.restype = d
+    L.oracle_qag.argtypes = [i, d, p, p, p, d, d, i]
+    L.oracle_qag.restype = d
+    L.oracle_force_divr.argtypes = [i, p, d, d, d, d]
+    L.oracle_force_divr.restype = d
+    L.oracle_energy.argtypes = [i, p, d, d, d, d]
+    L.oracle_energy.restype = d
+    L.oracle_ik_compute.argtypes = [ctypes.POINTER(_Args), ctypes.POINTER(_Out)]
+    L.oracle_ik_compute.restype = i
+    L.oracle_max_threads.restype = i
+    return L
 
 
 def _ptr(a):

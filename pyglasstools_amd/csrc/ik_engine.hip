@@ -219,6 +219,30 @@ struct DBuf {
     }
 };
 
+// Pinned host staging memory (hipHostMalloc), grown as needed.
+struct HostBuf {
+    unsigned char *p = nullptr;
+    size_t n = 0;
+    int ensure(size_t bytes)
+    {
+        if (bytes <= n && p) return 0;
+        release();
+        hipError_t e = hipHostMalloc((void **)&p, bytes ? bytes : 1, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(PGT_ERR_HIP, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        }
+        n = bytes ? bytes : 1;
+        return 0;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
 }  // namespace
 
 struct pgt_ik_engine {
@@ -238,14 +262,17 @@ struct pgt_ik_engine {
     DBuf<double> d_pos, d_diam, d_mass, d_vel, d_disp;  // single-frame host uploads
     DBuf<double> d_grid;
     DBuf<int> d_perm;
-    DBuf<int> d_keys0, d_vals0, d_keys1, d_vals1, d_hist, d_cstart, d_bcount, d_maxcount;  // cell list, bond counts
-    DBuf<double> d_spos, d_sdiam, d_skin, d_sdisp;
+    // work arrays of a batch (see WorkArray), as raw bytes
+    DBuf<unsigned char> d_keys0, d_vals0, d_keys1, d_vals1, d_hist, d_cstart, d_bcount;
+    DBuf<unsigned char> d_spos, d_sdiam, d_skin, d_sdisp;
     DBuf<unsigned char> d_bonds;
+    DBuf<int> d_maxcount;  // [0] largest bond count, [1] adaptive workspace ran out
     DBuf<double> d_out;
+    HostBuf h_stage;             // pinned staging of host-buffer uploads and downloads
+    std::vector<double> h_grid;  // the current grid (set_grid skips an unchanged one)
     DBuf<double> d_gl;
-    DBuf<unsigned long long> d_stats, d_wcounts;  // job totals; per-wavefront counts of a batch
-    DBuf<int> d_wmax;                             // per-wavefront largest bond count of a batch
-    DBuf<float4> d_rec;                           // 3D, per sorted particle: FP32 position, bond count
+    DBuf<unsigned long long> d_stats;              // job totals (balls, contributions)
+    DBuf<unsigned char> d_wcounts, d_wmax, d_rec;  // per-wavefront counts and maxima; 3D FP32 records
     // surrogate field form: pair counts and offsets per grid point, pair
     // records, network inputs and outputs, final bond integrals, counter
     DBuf<int> d_pcount, d_poff;
@@ -259,6 +286,12 @@ struct pgt_ik_engine {
     int frames_per_launch = 8;
     int points_per_lane = 2;  // grid points per gather lane (Octic 5-node rule; 1 elsewhere)
     int cell_reach = 1;       // cells searched on each side (cell edge >= search radius / reach)
+    int gather_mode = 0;      // 1: LDS-staged tile gather where the form has one, 0: one lane per point
+    // set by pgt_ik_enqueue_frames: launches that a captured graph may replay
+    // hold the current device buffers, so nothing may free or move them (or
+    // change the bond capacity the launches were built with) until
+    // pgt_ik_release_enqueued
+    bool held = false;
     pgt_ik_stats stats{};
 };
 
@@ -268,6 +301,13 @@ int check_engine(pgt_ik_engine *e)
 {
     if (!e) return fail(PGT_ERR_ARG, "null engine");
     return 0;
+}
+
+int fail_held()
+{
+    return fail(PGT_ERR_STATE, "device buffers are held by enqueued work that a captured graph may replay: call "
+                               "pgt_ik_release_enqueued before a call that would regrow or replace them, and capture "
+                               "again afterwards");
 }
 
 int ensure_events(pgt_ik_engine *e, size_t want)
@@ -421,92 +461,204 @@ size_t bond_entry_bytes(unsigned fields)
     return sizeof(ER) * BondLayout<D, OD, false, ER>::ES;
 }
 
-// Launch the cell list, bond list and gather for nf frames (no host sync).
-int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos, const double *diam,
-               const double *mass, const double *vel, const double *disp, unsigned fields, double *ovir,
-               double *okin, double *oela, bool z_flat, hipEvent_t ev_begin, hipEvent_t ev_gather, hipEvent_t ev_end,
-               bool no_alloc, bool bonds_only)
+// What a job's shape fixes about its batches: dimensions, fields, the cell
+// geometry and radix-sort passes, the bond-entry size and the gather shape.
+struct JobShape {
+    int D, OD, KN, PS;
+    bool pairs, kin, ela, fp32, adaptive;
+    Geom g;
+    int npass, nbits, radix;
+    size_t ebytes;
+    int block, pl;
+    bool tile;       // the LDS-staged tile gather (kTilePoints points per workgroup of kTileThreads lanes)
+    int64_t gwaves;  // gather wavefronts per frame
+};
+
+// allow_tile false: the lane gather even where the tile gather is tuned in
+// (the fallback when a tile window overflows the LDS arrays)
+JobShape job_shape(const pgt_ik_engine *e, unsigned fields, bool z_flat, bool allow_tile = true)
 {
-    // no_alloc: buffers must already be large enough (graph capture)
-    auto need = [&](auto &buf, size_t count) -> int {
-        if (!no_alloc) return buf.ensure(count);
-        if (buf.p && buf.n >= count) return 0;
-        return fail(PGT_ERR_STATE, "engine buffers are smaller than this job: run pgt_ik_compute_frames with the "
-                                   "same frame count and fields first");
-    };
-    const int dim = e->prm.dim;
-    const int D = (dim == 3 || !z_flat) ? 3 : 2;
-    const int OD = dim;
-    const int n = (int)n_particles;
-    const int G = (int)e->G;
-    const Geom g = make_geom(e, D);
-    const int KN = OD * (OD + 1) / 2;
-    const int PS = (D == 2) ? 2 : 4;
-    const bool pairs = fields & (PGT_FIELD_VIRIAL | PGT_FIELD_ELASTIC);
-    const bool kin = fields & PGT_FIELD_KINETIC;
-    const bool ela = fields & PGT_FIELD_ELASTIC;
+    JobShape j{};
+    j.OD = e->prm.dim;
+    j.D = (j.OD == 3 || !z_flat) ? 3 : 2;
+    j.KN = j.OD * (j.OD + 1) / 2;
+    j.PS = (j.D == 2) ? 2 : 4;
+    j.pairs = fields & (PGT_FIELD_VIRIAL | PGT_FIELD_ELASTIC);
+    j.kin = fields & PGT_FIELD_KINETIC;
+    j.ela = fields & PGT_FIELD_ELASTIC;
     // FP32 form: Octic kernel on the 2D (z == 0) and 3D paths; a 2D system
     // with nonzero z runs in FP64.
-    const bool fp32 = e->prm.precision == PGT_PREC_FP32 && !(D == 3 && OD == 2);
-    size_t ebytes;
-    if (D == 2) ebytes = fp32 ? bond_entry_bytes<2, 2, float>(fields) : bond_entry_bytes<2, 2, double>(fields);
-    else if (OD == 2) ebytes = bond_entry_bytes<3, 2, double>(fields);
-    else ebytes = fp32 ? bond_entry_bytes<3, 3, float>(fields) : bond_entry_bytes<3, 3, double>(fields);
-    int rc;
+    j.fp32 = e->prm.precision == PGT_PREC_FP32 && !(j.D == 3 && j.OD == 2);
+    j.adaptive = e->prm.quad_mode == PGT_QUAD_ADAPTIVE && j.pairs;
+    j.g = make_geom(e, j.D);
     // radix sort of the cell keys: digits of nbits bits, npass passes
     int kbits = 0;
-    while ((1 << kbits) < g.ncell) ++kbits;
-    const int npass = std::max(1, (kbits + 7) / 8);
-    const int nbits = std::max(1, (kbits + npass - 1) / npass);
-    const int radix = 1 << nbits;
+    while ((1 << kbits) < j.g.ncell) ++kbits;
+    j.npass = std::max(1, (kbits + 7) / 8);
+    j.nbits = std::max(1, (kbits + j.npass - 1) / j.npass);
+    j.radix = 1 << j.nbits;
+    if (j.D == 2) j.ebytes = j.fp32 ? bond_entry_bytes<2, 2, float>(fields) : bond_entry_bytes<2, 2, double>(fields);
+    else if (j.OD == 2) j.ebytes = bond_entry_bytes<3, 2, double>(fields);
+    else j.ebytes = j.fp32 ? bond_entry_bytes<3, 3, float>(fields) : bond_entry_bytes<3, 3, double>(fields);
+    j.block = j.adaptive ? 128 : e->block;  // k_gather_adaptive runs 128-thread workgroups
+    j.pl = j.adaptive ? 1 : gather_pl(e->points_per_lane, e->prm.cg_kind, (int)e->gl_s.size(), j.D, j.OD);
+    j.tile = allow_tile && !j.adaptive && e->gather_mode == 1 &&
+             tile_form(e->prm.cg_kind, (int)e->gl_s.size(), j.D, j.OD);
+    if (j.tile) {
+        j.block = kTileThreads;
+        j.pl = 1;
+        j.gwaves = (e->G + kTilePoints - 1) / kTilePoints * (kTileThreads / 64);
+    } else {
+        const int64_t lanes = (e->G + j.pl - 1) / j.pl;
+        j.gwaves = (lanes + j.block - 1) / j.block * (j.block / 64);
+    }
+    return j;
+}
+
+// Device work arrays of one batch of frames, and the job's counters.
+enum WorkArray {
+    W_KEYS0, W_VALS0, W_KEYS1, W_VALS1, W_HIST, W_CSTART, W_SPOS, W_SDIAM, W_BCOUNT, W_SKIN, W_SDISP, W_REC,
+    W_BONDS, W_WCOUNTS, W_WMAX, W_NARRAYS
+};
+
+struct Work {
+    void *p[W_NARRAYS];
+    unsigned long long *stats;  // job totals: balls, contributions
+    int *maxcount;              // [0] largest bond count, [1] an adaptive integral ran out of workspace
+    template <typename T>
+    T *get(WorkArray a) const { return static_cast<T *>(p[a]); }
+};
+
+// Bytes of each work array for nf frames of n particles at bond capacity cap
+// (0 for arrays the job does not use).
+void work_bytes(const JobShape &j, int nf, int64_t n, int64_t G, int cap, size_t out[W_NARRAYS])
+{
+    const size_t fn = (size_t)nf * (size_t)n;
+    const size_t nchunk = (size_t)((n + kSortChunk - 1) / kSortChunk);
+    const size_t nw_gather_max = (size_t)std::max<int64_t>(j.gwaves, (G + j.block - 1) / j.block * (j.block / 64)) * nf;
+    const size_t nw_bonds = j.pairs ? (size_t)((n + kBondBlock - 1) / kBondBlock) * nf * (kBondBlock / 64) : 0;
+    for (int k = 0; k < W_NARRAYS; ++k) out[k] = 0;
+    out[W_KEYS0] = out[W_VALS0] = out[W_KEYS1] = out[W_VALS1] = fn * sizeof(int);
+    out[W_HIST] = (size_t)nf * j.radix * std::max<size_t>(nchunk, 1) * sizeof(int);
+    out[W_CSTART] = (size_t)nf * (j.g.ncell + 1) * sizeof(int);
+    out[W_SPOS] = fn * j.PS * sizeof(double);
+    out[W_SDIAM] = fn * sizeof(double);
+    out[W_BCOUNT] = fn * sizeof(int);
+    if (j.kin) out[W_SKIN] = fn * j.KN * sizeof(double);
+    if (j.ela) out[W_SDISP] = fn * 3 * sizeof(double);
+    if (j.pairs && j.D == 3) out[W_REC] = fn * sizeof(float4);
+    if (j.pairs) out[W_BONDS] = fn * (size_t)cap * j.ebytes;
+    // per-wavefront statistics, sized for one point per gather lane (the most
+    // wavefronts any shape launches)
+    out[W_WCOUNTS] = 2 * std::max<size_t>(nw_gather_max, 1) * sizeof(unsigned long long);
+    out[W_WMAX] = std::max<size_t>(nw_bonds, 1) * sizeof(int);
+}
+
+constexpr size_t kWorkAlign = 256;
+
+size_t align_up(size_t b) { return (b + kWorkAlign - 1) / kWorkAlign * kWorkAlign; }
+
+// Work arrays from the engine's own buffers, grown as needed unless no_alloc
+// (graph capture: the buffers must already be large enough).
+int work_from_engine(pgt_ik_engine *e, const JobShape &j, int nf, int64_t n, bool no_alloc, Work &w)
+{
+    size_t b[W_NARRAYS];
+    work_bytes(j, nf, n, e->G, e->cap, b);
+    DBuf<unsigned char> *bufs[W_NARRAYS] = {&e->d_keys0, &e->d_vals0, &e->d_keys1, &e->d_vals1, &e->d_hist,
+                                            &e->d_cstart, &e->d_spos, &e->d_sdiam, &e->d_bcount, &e->d_skin,
+                                            &e->d_sdisp, &e->d_rec, &e->d_bonds, &e->d_wcounts, &e->d_wmax};
+    for (int k = 0; k < W_NARRAYS; ++k) {
+        w.p[k] = nullptr;
+        if (!b[k]) continue;
+        DBuf<unsigned char> &buf = *bufs[k];
+        if (!(buf.p && buf.n >= b[k])) {
+            if (no_alloc)
+                return fail(PGT_ERR_STATE, "engine buffers are smaller than this job: run pgt_ik_compute_frames with "
+                                           "the same frame count and fields first");
+            if (e->held) return fail_held();
+            if (int rc = buf.ensure(b[k])) return rc;
+        }
+        w.p[k] = buf.p;
+    }
+    w.stats = e->d_stats.p;
+    w.maxcount = e->d_maxcount.p;
+    return 0;
+}
+
+// Work arrays carved out of `base` (the no-allocation entry point's scratch
+// array): the job counters first, then each array, all 256-byte aligned.
+// Returns the bytes used.
+size_t work_carve(const JobShape &j, int nf, int64_t n, int64_t G, int cap, unsigned char *base, Work *w)
+{
+    size_t b[W_NARRAYS];
+    work_bytes(j, nf, n, G, cap, b);
+    size_t off = 0;
+    if (w) {
+        w->stats = reinterpret_cast<unsigned long long *>(base);
+        w->maxcount = reinterpret_cast<int *>(base + 2 * sizeof(unsigned long long));
+    }
+    off = align_up(2 * sizeof(unsigned long long) + 2 * sizeof(int));
+    for (int k = 0; k < W_NARRAYS; ++k) {
+        if (w) w->p[k] = b[k] ? base + off : nullptr;
+        off += align_up(b[k]);
+    }
+    return off;
+}
+
+// Launch the cell list, bond list and gather for nf frames (no host sync).
+int run_frames(pgt_ik_engine *e, const JobShape &j, const Work &w, int cap, int nf, int64_t n_particles,
+               const double *pos, const double *diam, const double *mass, const double *vel, const double *disp,
+               unsigned fields, double *ovir, double *okin, double *oela, hipEvent_t ev_begin, hipEvent_t ev_gather,
+               hipEvent_t ev_end, bool bonds_only)
+{
+    const int D = j.D, OD = j.OD;
+    const int n = (int)n_particles;
+    const int G = (int)e->G;
+    const Geom &g = j.g;
     const int nchunk = (n + kSortChunk - 1) / kSortChunk;
-    if ((rc = need(e->d_keys0, (size_t)nf * n)) || (rc = need(e->d_vals0, (size_t)nf * n)) ||
-        (rc = need(e->d_keys1, (size_t)nf * n)) || (rc = need(e->d_vals1, (size_t)nf * n)) ||
-        (rc = need(e->d_hist, (size_t)nf * radix * std::max(nchunk, 1))) ||
-        (rc = need(e->d_cstart, (size_t)nf * (g.ncell + 1))) || (rc = need(e->d_spos, (size_t)nf * n * PS)) ||
-        (rc = need(e->d_sdiam, (size_t)nf * n)) || (rc = need(e->d_bcount, (size_t)nf * n)))
-        return rc;
-    if (kin && (rc = need(e->d_skin, (size_t)nf * n * KN))) return rc;
-    if (ela && (rc = need(e->d_sdisp, (size_t)nf * n * 3))) return rc;
-    if (pairs && D == 3 && (rc = need(e->d_rec, (size_t)nf * n))) return rc;
-    float4 *rec = pairs && D == 3 ? e->d_rec.p : nullptr;
+    float4 *rec = w.get<float4>(W_REC);
+    double *skin = w.get<double>(W_SKIN);
+    double *sdisp = w.get<double>(W_SDISP);
+    int *hist = w.get<int>(W_HIST);
+    int *cstart = w.get<int>(W_CSTART);
+    double *spos = w.get<double>(W_SPOS);
+    double *sdiam = w.get<double>(W_SDIAM);
+    int *bcount = w.get<int>(W_BCOUNT);
     hipStream_t s = e->stream;
     if (ev_begin) HIPCHK(hipEventRecord(ev_begin, s));
-    double *skin = kin ? e->d_skin.p : nullptr;
-    double *sdisp = ela ? e->d_sdisp.p : nullptr;
     // cell list: per pass a histogram, a scan and a scatter of one digit of
     // the cell keys; frames are dealt to XCDs whole (frame_block)
     {
         const dim3 cgrid((unsigned)nchunk, (unsigned)frame_rows(nf));
-        DBuf<int> *kb[2] = {&e->d_keys0, &e->d_keys1}, *vb[2] = {&e->d_vals0, &e->d_vals1};
-        for (int p = 0; p < npass; ++p) {
-            const int shift = p * nbits;
-            int *kin = kb[p & 1]->p, *vin = vb[p & 1]->p, *kout = kb[(p + 1) & 1]->p, *vout = vb[(p + 1) & 1]->p;
+        int *kb[2] = {w.get<int>(W_KEYS0), w.get<int>(W_KEYS1)}, *vb[2] = {w.get<int>(W_VALS0), w.get<int>(W_VALS1)};
+        for (int p = 0; p < j.npass; ++p) {
+            const int shift = p * j.nbits, radix = j.radix;
+            int *kin = kb[p & 1], *vin = vb[p & 1], *kout = kb[(p + 1) & 1], *vout = vb[(p + 1) & 1];
             if (p == 0) {
                 hipLaunchKernelGGL(k_sort_hist<true>, cgrid, dim3(kSortBlock), 0, s, g, n, nf, shift, radix, pos, kin,
-                                   e->d_hist.p);
-                hipLaunchKernelGGL(k_sort_scan, dim3(nf), dim3(1024), 0, s, radix * nchunk, e->d_hist.p);
+                                   hist);
+                hipLaunchKernelGGL(k_sort_scan, dim3(nf), dim3(1024), 0, s, radix * nchunk, hist);
                 hipLaunchKernelGGL(k_sort_scatter<true>, cgrid, dim3(kSortBlock), 0, s, n, nf, shift, radix, kin, vin,
-                                   e->d_hist.p, kout, vout);
+                                   hist, kout, vout);
             } else {
                 hipLaunchKernelGGL(k_sort_hist<false>, cgrid, dim3(kSortBlock), 0, s, g, n, nf, shift, radix, pos,
-                                   kin, e->d_hist.p);
-                hipLaunchKernelGGL(k_sort_scan, dim3(nf), dim3(1024), 0, s, radix * nchunk, e->d_hist.p);
+                                   kin, hist);
+                hipLaunchKernelGGL(k_sort_scan, dim3(nf), dim3(1024), 0, s, radix * nchunk, hist);
                 hipLaunchKernelGGL(k_sort_scatter<false>, cgrid, dim3(kSortBlock), 0, s, n, nf, shift, radix, kin,
-                                   vin, e->d_hist.p, kout, vout);
+                                   vin, hist, kout, vout);
             }
         }
-        const int *keys = kb[npass & 1]->p, *vals = vb[npass & 1]->p;
+        const int *keys = kb[j.npass & 1], *vals = vb[j.npass & 1];
         const dim3 pgrid((unsigned)((n + 255) / 256), (unsigned)frame_rows(nf));
         if (D == 2)
             hipLaunchKernelGGL((k_sort_finish<2, 2>), pgrid, dim3(256), 0, s, g, n, nf, g.ncell, keys, vals, pos, diam,
-                               mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp, rec);
+                               mass, vel, disp, cstart, spos, sdiam, skin, sdisp, rec);
         else if (OD == 2)
             hipLaunchKernelGGL((k_sort_finish<3, 2>), pgrid, dim3(256), 0, s, g, n, nf, g.ncell, keys, vals, pos, diam,
-                               mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp, rec);
+                               mass, vel, disp, cstart, spos, sdiam, skin, sdisp, rec);
         else
             hipLaunchKernelGGL((k_sort_finish<3, 3>), pgrid, dim3(256), 0, s, g, n, nf, g.ncell, keys, vals, pos, diam,
-                               mass, vel, disp, e->d_cstart.p, e->d_spos.p, e->d_sdiam.p, skin, sdisp, rec);
+                               mass, vel, disp, cstart, spos, sdiam, skin, sdisp, rec);
     }
     HIPCHK(hipGetLastError());
 
@@ -515,8 +667,7 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     c.n = n;
     c.G = G;
     c.nf = nf;
-    const bool adaptive = e->prm.quad_mode == PGT_QUAD_ADAPTIVE && pairs;
-    c.block = adaptive ? 128 : e->block;  // k_gather_adaptive runs 128-thread workgroups
+    c.block = j.block;
     c.grid = e->d_grid.p;
     c.perm = e->d_perm.p;
     c.R.n = (int)e->gl_s.size();
@@ -526,72 +677,69 @@ int run_frames(pgt_ik_engine *e, int nf, int64_t n_particles, const double *pos,
     }
     c.gld = e->d_gl.p;
     c.cgnorm = cg_norm(e->prm.cg_kind, e->prm.cg_rcut);
-    c.cstart = e->d_cstart.p;
-    c.spos = e->d_spos.p;
+    c.cstart = cstart;
+    c.spos = spos;
     c.skin = skin;
     c.out.vir = ovir;
     c.out.kin = okin;
     c.out.ela = oela;
+    c.flags = w.maxcount + 1;
+    c.tile = j.tile ? 1 : 0;
     c.stream = s;
 
     // per-wavefront statistics of this batch (see k_reduce_stats)
-    c.pl = adaptive ? 1 : gather_pl(e->points_per_lane, e->prm.cg_kind, (int)e->gl_s.size(), D, OD);
-    const size_t lanes_g = ((size_t)G + c.pl - 1) / c.pl;  // gather lanes per frame
-    const int nw_gather = (int)((lanes_g + c.block - 1) / c.block) * nf * (c.block / 64);
-    // sized for one point per lane, the most wavefronts any shape launches
-    const int nw_gather_max = (int)(((size_t)G + c.block - 1) / c.block) * nf * (c.block / 64);
-    const int nw_bonds = pairs ? (int)(((size_t)n + kBondBlock - 1) / kBondBlock) * nf * (kBondBlock / 64) : 0;
-    if ((rc = need(e->d_wcounts, 2 * (size_t)std::max(nw_gather_max, 1))) || (rc = need(e->d_wmax, (size_t)std::max(nw_bonds, 1))))
-        return rc;
-    c.out.wave_counts = e->d_wcounts.p;
-    if (pairs) {
+    c.pl = j.pl;
+    const int nw_gather = (int)(j.gwaves * nf);
+    const int nw_bonds = j.pairs ? (int)(((size_t)n + kBondBlock - 1) / kBondBlock) * nf * (kBondBlock / 64) : 0;
+    c.out.wave_counts = w.get<unsigned long long>(W_WCOUNTS);
+    int *wmax = w.get<int>(W_WMAX);
+    void *bonds = w.p[W_BONDS];
+    if (j.pairs) {
         // entries beyond the capacity are counted, not written; the caller
-        // checks d_maxcount after the job and re-runs with a larger capacity
-        if (e->cap < 1) e->cap = D == 3 ? 32 : 16;
-        if ((rc = need(e->d_bonds, (size_t)nf * n * (size_t)e->cap * ebytes))) return rc;
-        c.cap = e->cap;
+        // checks maxcount after the job and re-runs with a larger capacity
+        c.cap = cap;
         const PotParams P = make_pot(e->prm);
         if (D == 2) {
-            if (fp32) launch_bonds_t<2, 2, float>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p, e->d_rec.p);
-            else launch_bonds_t<2, 2, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p, e->d_rec.p);
+            if (j.fp32) launch_bonds_t<2, 2, float>(c, fields, P, sdiam, sdisp, bcount, bonds, wmax, rec);
+            else launch_bonds_t<2, 2, double>(c, fields, P, sdiam, sdisp, bcount, bonds, wmax, rec);
         } else if (OD == 2) {
-            launch_bonds_t<3, 2, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p, e->d_rec.p);
+            launch_bonds_t<3, 2, double>(c, fields, P, sdiam, sdisp, bcount, bonds, wmax, rec);
         } else {
-            if (fp32) launch_bonds_t<3, 3, float>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p, e->d_rec.p);
-            else launch_bonds_t<3, 3, double>(c, fields, P, e->d_sdiam.p, sdisp, e->d_bcount.p, e->d_bonds.p, e->d_wmax.p, e->d_rec.p);
+            if (j.fp32) launch_bonds_t<3, 3, float>(c, fields, P, sdiam, sdisp, bcount, bonds, wmax, rec);
+            else launch_bonds_t<3, 3, double>(c, fields, P, sdiam, sdisp, bcount, bonds, wmax, rec);
         }
         HIPCHK(hipGetLastError());
     } else {
         c.cap = 1;
     }
-    c.bcount = e->d_bcount.p;
+    c.bcount = bcount;
     c.rec = rec;
-    c.bonds = e->d_bonds.p;
+    c.bonds = bonds;
     c.pack = n < (1 << (c.pl > 1 ? 22 : 26)) && c.cap <= 63;
     if (ev_gather) HIPCHK(hipEventRecord(ev_gather, s));
     const int cg = e->prm.cg_kind;
     if (bonds_only) {
         // cell list and bond list only (the surrogate field form reads them)
-    } else if (adaptive) {
+    } else if (j.adaptive) {
         const AdaptQ Q{e->prm.adapt_limit, e->prm.adapt_epsabs, e->prm.adapt_epsrel};
-        launch_gather_adaptive(c, fields, cg, D, OD, Q, e->d_maxcount.p + 1);
+        launch_gather_adaptive(c, fields, cg, D, OD, Q, c.flags);
     } else if (D == 2) {
-        if (fp32) launch_gather_2d32(c, fields, cg);
+        if (j.fp32) launch_gather_2d32(c, fields, cg);
         else launch_gather_2d64(c, fields, cg);
     } else if (OD == 2) {
         launch_gather_3d2(c, fields, cg);
     } else {
-        if (fp32) launch_gather_3d32(c, fields, cg);
+        if (j.fp32) launch_gather_3d32(c, fields, cg);
         else launch_gather_3d64(c, fields, cg);
     }
     HIPCHK(hipGetLastError());
-    if (!bonds_only && !adaptive && c.pl_launched != c.pl)
+    if (!bonds_only && !j.adaptive && c.pl_launched != c.pl)
         return fail(PGT_ERR_STATE, "gather launched with an unexpected number of points per lane");
     if (ev_end) HIPCHK(hipEventRecord(ev_end, s));
     const int nwg = bonds_only ? 0 : nw_gather;
     const int nrb = std::max(1, std::min(256, (std::max(nwg, nw_bonds) + 4095) / 4096));
-    hipLaunchKernelGGL(k_reduce_stats, dim3(nrb), dim3(1024), 0, s, nwg, e->d_wcounts.p, nw_bonds, e->d_wmax.p,
-                       e->d_stats.p, e->d_maxcount.p);
+    hipLaunchKernelGGL(k_reduce_stats, dim3(nrb), dim3(1024), 0, s, nwg, c.out.wave_counts, nw_bonds, wmax, w.stats,
+                       w.maxcount);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -605,43 +753,59 @@ struct FrameSet {
     bool bonds_only;  // build the cell and bond lists only (no field gather)
 };
 
+// Launch batch f0 .. f0 + nf - 1 of a job with work arrays w.
+int run_batch(pgt_ik_engine *e, const JobShape &j, const Work &w, int cap, const FrameSet &fs, unsigned fields, int f0,
+              int nf, hipEvent_t e0, hipEvent_t e1, hipEvent_t e2)
+{
+    const size_t fsz = (size_t)e->G * j.OD * j.OD;
+    const size_t po = (size_t)f0 * fs.n;
+    return run_frames(e, j, w, cap, nf, fs.n, fs.pos + po * 3, fs.diam + po, fs.mass ? fs.mass + po : nullptr,
+                      fs.vel ? fs.vel + po * 3 : nullptr, fs.disp ? fs.disp + po * 3 : nullptr, fields,
+                      fs.vir ? fs.vir + (size_t)f0 * fsz : nullptr, fs.kin ? fs.kin + (size_t)f0 * fsz : nullptr,
+                      fs.ela ? fs.ela + (size_t)f0 * fsz : nullptr, e0, e1, e2, fs.bonds_only);
+}
+
 // Run every batch of a job on the engine's stream, then check the bond-list
 // capacity once; on overflow grow it and run the whole job again.
 int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
 {
-    const int OD = e->prm.dim;
-    const size_t fsz = (size_t)e->G * OD * OD;
     const int fpl = std::max(1, e->frames_per_launch);
     const int nb = (fs.nframes + fpl - 1) / fpl;
+    JobShape j = job_shape(e, fields, fs.z_flat);
     int rc;
     if ((rc = ensure_events(e, (size_t)nb * 3))) return rc;
     if ((rc = e->d_maxcount.ensure(2)) || (rc = e->d_stats.ensure(2))) return rc;
+    if (j.pairs && e->cap < 1) e->cap = j.D == 3 ? 32 : 16;
     int mx = 0;
     for (int attempt = 0; attempt < 3; ++attempt) {
-        // [0] largest bond count, [1] an adaptive integral ran out of workspace
-        if ((rc = fill_async(e->d_maxcount.p, 0, 2, e->stream)) || (rc = fill_async(e->d_stats.p, 0ull, 2, e->stream)))
-            return rc;
+        Work w{};
+        if ((rc = work_from_engine(e, j, std::min(fpl, fs.nframes), fs.n, false, w))) return rc;
+        if ((rc = fill_async(w.maxcount, 0, 2, e->stream)) || (rc = fill_async(w.stats, 0ull, 2, e->stream))) return rc;
         for (int b = 0; b < nb; ++b) {
             const int f0 = b * fpl;
-            const int nf = std::min(fpl, fs.nframes - f0);
-            const size_t po = (size_t)f0 * fs.n;
-            rc = run_frames(e, nf, fs.n, fs.pos + po * 3, fs.diam + po, fs.mass ? fs.mass + po : nullptr,
-                            fs.vel ? fs.vel + po * 3 : nullptr, fs.disp ? fs.disp + po * 3 : nullptr, fields,
-                            fs.vir ? fs.vir + (size_t)f0 * fsz : nullptr, fs.kin ? fs.kin + (size_t)f0 * fsz : nullptr,
-                            fs.ela ? fs.ela + (size_t)f0 * fsz : nullptr, fs.z_flat, e->ev[3 * b],
-                            e->ev[3 * b + 1], e->ev[3 * b + 2], false, fs.bonds_only);
+            rc = run_batch(e, j, w, e->cap, fs, fields, f0, std::min(fpl, fs.nframes - f0), e->ev[3 * b],
+                           e->ev[3 * b + 1], e->ev[3 * b + 2]);
             if (rc) return rc;
         }
         int flags[2] = {0, 0};
         HIPCHK(hipMemcpyAsync(flags, e->d_maxcount.p, sizeof(flags), hipMemcpyDeviceToHost, e->stream));
         HIPCHK(hipStreamSynchronize(e->stream));
         mx = flags[0];
-        if (flags[1])
+        if (flags[1] & 1)
             return fail(PGT_ERR_RANGE, "an adaptive bond integral needed more than " + std::to_string(kQagMax) +
                                            " intervals (the GPU rule's workspace)");
-        if (mx <= e->cap) break;
-        if (mx > 65535) return fail(PGT_ERR_RANGE, "a particle has more than 65535 interacting neighbours");
-        e->cap = ((mx + 7) / 8) * 8;
+        const bool grow = j.pairs && mx > e->cap;
+        if (grow && mx > 65535) return fail(PGT_ERR_RANGE, "a particle has more than 65535 interacting neighbours");
+        if (grow && e->held) return fail_held();  // the graph's launches keep the old capacity
+        if (flags[1] & 6) {
+            // a tile window held more particles or entries than the LDS
+            // arrays: run the job again with the lane gather
+            if (!j.tile) return fail(PGT_ERR_STATE, "tile overflow reported by a lane gather");
+            j = job_shape(e, fields, fs.z_flat, false);
+        } else if (!grow) {
+            break;
+        }
+        if (grow) e->cap = ((mx + 7) / 8) * 8;
         if (attempt == 2) return fail(PGT_ERR_HIP, "bond list capacity did not settle");
     }
     unsigned long long h[2] = {0, 0};
@@ -671,22 +835,17 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
 // enqueued work shows up in pgt_ik_finish.
 int enqueue_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
 {
-    const int OD = e->prm.dim;
-    const size_t fsz = (size_t)e->G * OD * OD;
     const int fpl = std::max(1, e->frames_per_launch);
     const int nb = (fs.nframes + fpl - 1) / fpl;
     if (e->cap < 1 || !e->d_maxcount.p || e->d_maxcount.n < 2 || !e->d_stats.p)
         return fail(PGT_ERR_STATE, "run pgt_ik_compute_frames of the same shape before enqueueing");
+    const JobShape j = job_shape(e, fields, fs.z_flat);
+    Work w{};
+    if (int rc = work_from_engine(e, j, std::min(fpl, fs.nframes), fs.n, true, w)) return rc;
     for (int b = 0; b < nb; ++b) {
         const int f0 = b * fpl;
-        const int nf = std::min(fpl, fs.nframes - f0);
-        const size_t po = (size_t)f0 * fs.n;
-        const int rc = run_frames(e, nf, fs.n, fs.pos + po * 3, fs.diam + po, fs.mass ? fs.mass + po : nullptr,
-                                  fs.vel ? fs.vel + po * 3 : nullptr, fs.disp ? fs.disp + po * 3 : nullptr, fields,
-                                  fs.vir ? fs.vir + (size_t)f0 * fsz : nullptr,
-                                  fs.kin ? fs.kin + (size_t)f0 * fsz : nullptr,
-                                  fs.ela ? fs.ela + (size_t)f0 * fsz : nullptr, fs.z_flat, nullptr, nullptr, nullptr,
-                                  true, false);
+        const int rc = run_batch(e, j, w, e->cap, fs, fields, f0, std::min(fpl, fs.nframes - f0), nullptr, nullptr,
+                                 nullptr);
         if (rc) return rc;
     }
     return 0;
@@ -749,8 +908,11 @@ int prepare_pairs(pgt_ik_engine *e, int64_t *np, bool write, float *xin)
     if ((rc = e->d_pcount.ensure(G)) || (rc = e->d_poff.ensure(G)) || (rc = e->d_pn.ensure(2))) return rc;
     if ((rc = fill_async(e->d_pn.p + 1, 0ull, 1, s))) return rc;
     const dim3 gg((unsigned)((G + 127) / 128));
-    hipLaunchKernelGGL(k_pair_count, gg, dim3(128), 0, s, g, G, e->d_grid.p, e->d_cstart.p, e->d_spos.p, e->cap,
-                       e->d_bcount.p, e->d_pcount.p, e->d_pn.p + 1);
+    const int *cstart = reinterpret_cast<const int *>(e->d_cstart.p);
+    const double *spos = reinterpret_cast<const double *>(e->d_spos.p);
+    const int *bcount = reinterpret_cast<const int *>(e->d_bcount.p);
+    hipLaunchKernelGGL(k_pair_count, gg, dim3(128), 0, s, g, G, e->d_grid.p, cstart, spos, e->cap, bcount,
+                       e->d_pcount.p, e->d_pn.p + 1);
     unsigned long long total = 0;
     HIPCHK(hipMemcpyAsync(&total, e->d_pn.p + 1, sizeof(total), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -765,8 +927,8 @@ int prepare_pairs(pgt_ik_engine *e, int64_t *np, bool write, float *xin)
         if ((rc = e->d_pxin.ensure((size_t)*np * kMlpInPad))) return rc;
         xin = e->d_pxin.p;
     }
-    hipLaunchKernelGGL(k_pair_write, gg, dim3(128), 0, s, g, G, e->d_grid.p, e->d_cstart.p, e->d_spos.p, e->cap,
-                       e->d_bcount.p, (const double *)e->d_bonds.p, e->d_poff.p, e->prm.cg_rcut, e->d_prec.p, xin);
+    hipLaunchKernelGGL(k_pair_write, gg, dim3(128), 0, s, g, G, e->d_grid.p, cstart, spos, e->cap, bcount,
+                       (const double *)e->d_bonds.p, e->d_poff.p, e->prm.cg_rcut, e->d_prec.p, xin);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -833,12 +995,14 @@ void pgt_ik_destroy(pgt_ik_engine *e)
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     DBuf<double> *dbufs[] = {&e->d_pos, &e->d_diam, &e->d_mass, &e->d_vel, &e->d_disp, &e->d_grid,
-                             &e->d_spos, &e->d_sdiam, &e->d_skin, &e->d_sdisp, &e->d_out, &e->d_gl};
+                             &e->d_out, &e->d_gl};
     for (auto *b : dbufs) b->release();
-    DBuf<int> *ibufs[] = {&e->d_perm,  &e->d_keys0,  &e->d_vals0,  &e->d_keys1,   &e->d_vals1,
-                          &e->d_hist, &e->d_cstart, &e->d_bcount, &e->d_maxcount};
+    DBuf<int> *ibufs[] = {&e->d_perm, &e->d_maxcount};
     for (auto *b : ibufs) b->release();
-    e->d_bonds.release();
+    DBuf<unsigned char> *wbufs[] = {&e->d_keys0, &e->d_vals0, &e->d_keys1, &e->d_vals1, &e->d_hist, &e->d_cstart,
+                                    &e->d_bcount, &e->d_spos, &e->d_sdiam, &e->d_skin, &e->d_sdisp, &e->d_bonds,
+                                    &e->d_wcounts, &e->d_wmax, &e->d_rec};
+    for (auto *b : wbufs) b->release();
     e->d_pcount.release();
     e->d_poff.release();
     e->d_prec.release();
@@ -847,9 +1011,7 @@ void pgt_ik_destroy(pgt_ik_engine *e)
     e->d_pw.release();
     e->d_pn.release();
     e->d_stats.release();
-    e->d_wcounts.release();
-    e->d_wmax.release();
-    e->d_rec.release();
+    e->h_stage.release();
     for (hipEvent_t v : e->ev) (void)hipEventDestroy(v);
     if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
     delete e;
@@ -883,25 +1045,30 @@ int pgt_ik_set_particles(pgt_ik_engine *e, int64_t n, const double *pos, const d
     if (n > 0 && (!pos || !diam)) return fail(PGT_ERR_ARG, "positions and diameters are required");
     int rc;
     if ((rc = e->d_pos.ensure((size_t)n * 3)) || (rc = e->d_diam.ensure((size_t)n))) return rc;
+    if (mass && n > 0 && (rc = e->d_mass.ensure((size_t)n))) return rc;
+    if (vel && n > 0 && (rc = e->d_vel.ensure((size_t)n * 3))) return rc;
+    if (disp && n > 0 && (rc = e->d_disp.ensure((size_t)n * 3))) return rc;
     hipStream_t s = e->stream;
-    if (n > 0) {
-        HIPCHK(hipMemcpyAsync(e->d_pos.p, pos, sizeof(double) * n * 3, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(e->d_diam.p, diam, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    }
     e->have_mass = mass != nullptr;
     e->have_vel = vel != nullptr;
     e->have_disp = disp != nullptr;
-    if (mass && n > 0) {
-        if ((rc = e->d_mass.ensure((size_t)n))) return rc;
-        HIPCHK(hipMemcpyAsync(e->d_mass.p, mass, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    }
-    if (vel && n > 0) {
-        if ((rc = e->d_vel.ensure((size_t)n * 3))) return rc;
-        HIPCHK(hipMemcpyAsync(e->d_vel.p, vel, sizeof(double) * n * 3, hipMemcpyHostToDevice, s));
-    }
-    if (disp && n > 0) {
-        if ((rc = e->d_disp.ensure((size_t)n * 3))) return rc;
-        HIPCHK(hipMemcpyAsync(e->d_disp.p, disp, sizeof(double) * n * 3, hipMemcpyHostToDevice, s));
+    if (n > 0) {
+        // through pinned staging memory: one host copy, then DMA at full rate
+        const double *src[5] = {pos, diam, mass, vel, disp};
+        double *dst[5] = {e->d_pos.p, e->d_diam.p, e->d_mass.p, e->d_vel.p, e->d_disp.p};
+        const size_t cnt[5] = {3, 1, 1, 3, 3};
+        size_t tot = 0;
+        for (int k = 0; k < 5; ++k)
+            if (src[k]) tot += cnt[k] * (size_t)n;
+        if ((rc = e->h_stage.ensure(tot * sizeof(double)))) return rc;
+        double *h = reinterpret_cast<double *>(e->h_stage.p);
+        for (int k = 0; k < 5; ++k) {
+            if (!src[k]) continue;
+            const size_t bytes = sizeof(double) * cnt[k] * (size_t)n;
+            std::memcpy(h, src[k], bytes);
+            HIPCHK(hipMemcpyAsync(dst[k], h, bytes, hipMemcpyHostToDevice, s));
+            h += cnt[k] * (size_t)n;
+        }
     }
     bool flat = true;
     for (int64_t i = 0; i < n && flat; ++i) flat = pos[3 * i + 2] == 0.0;
@@ -917,6 +1084,11 @@ int pgt_ik_set_grid(pgt_ik_engine *e, int64_t g, const double *grid)
     if (int rc = check_engine(e)) return rc;
     if (g < 0 || g > (int64_t)1 << 30) return fail(PGT_ERR_ARG, "grid size out of range");
     if (g > 0 && !grid) return fail(PGT_ERR_ARG, "null grid");
+    // the grid already set (a driver passes the same grid on every frame):
+    // no Morton sort, upload or synchronisation
+    if (e->have_grid && g == e->G && (g == 0 || std::memcmp(e->h_grid.data(), grid, sizeof(double) * 3 * g) == 0))
+        return PGT_OK;
+    if (e->held) return fail_held();  // enqueued launches read the grid with its size built in
     bool flat = true;
     double amax = 0.0;
     for (int64_t i = 0; i < g; ++i) {
@@ -932,6 +1104,7 @@ int pgt_ik_set_grid(pgt_ik_engine *e, int64_t g, const double *grid)
         HIPCHK(hipMemcpyAsync(e->d_perm.p, perm.data(), sizeof(int) * g, hipMemcpyHostToDevice, e->stream));
     }
     HIPCHK(hipStreamSynchronize(e->stream));
+    e->h_grid.assign(grid, grid + 3 * g);
     e->grid_z_flat = flat;
     e->grid_absmax = amax;
     e->G = g;
@@ -974,7 +1147,16 @@ int pgt_ik_enqueue_frames(pgt_ik_engine *e, int nframes, int64_t n, const double
     fs.kin = (fields & PGT_FIELD_KINETIC) ? d_out_kinetic : nullptr;
     fs.ela = (fields & PGT_FIELD_ELASTIC) ? d_out_elastic : nullptr;
     fs.z_flat = e->grid_z_flat;
-    return enqueue_job(e, fs, fields);
+    const int rc = enqueue_job(e, fs, fields);
+    if (rc == PGT_OK) e->held = true;
+    return rc;
+}
+
+int pgt_ik_release_enqueued(pgt_ik_engine *e)
+{
+    if (int rc = check_engine(e)) return rc;
+    e->held = false;
+    return PGT_OK;
 }
 
 int pgt_ik_finish(pgt_ik_engine *e)
@@ -994,9 +1176,13 @@ int pgt_ik_finish(pgt_ik_engine *e)
     e->stats.n_contrib = (int64_t)h[1];
     e->stats.max_bonds = flags[0];
     e->stats.bond_capacity = e->cap;
-    if (flags[1])
+    if (flags[1] & 1)
         return fail(PGT_ERR_RANGE, "an adaptive bond integral needed more than " + std::to_string(kQagMax) +
                                        " intervals (the GPU rule's workspace)");
+    if (flags[1] & 6)
+        return fail(PGT_ERR_RANGE, "a tile of the gather held more particles or bond entries than its LDS arrays: "
+                                   "the enqueued results are incomplete; select the lane gather "
+                                   "(pgt_ik_set_gather_mode) and capture again");
     if (flags[0] > e->cap)
         return fail(PGT_ERR_RANGE, "a particle had more bonds than the capacity sized by the last "
                                    "pgt_ik_compute_frames: the enqueued results are incomplete; run "
@@ -1034,6 +1220,14 @@ int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_la
         if (cell_reach != 1 && cell_reach != 2) return fail(PGT_ERR_ARG, "cell reach must be 1 or 2");
         e->cell_reach = cell_reach;
     }
+    return PGT_OK;
+}
+
+int pgt_ik_set_gather_mode(pgt_ik_engine *e, int mode)
+{
+    if (int rc = check_engine(e)) return rc;
+    if (mode != 0 && mode != 1) return fail(PGT_ERR_ARG, "gather mode must be 0 (lane) or 1 (tile)");
+    e->gather_mode = mode;
     return PGT_OK;
 }
 
@@ -1079,9 +1273,18 @@ int pgt_ik_compute(pgt_ik_engine *e, unsigned fields, double *virial, double *ki
     fs.ela = e->d_out.p + 2 * fsz;
     fs.z_flat = e->z_flat && e->grid_z_flat;
     if ((rc = run_job(e, fs, fields))) return rc;
-    if (fields & PGT_FIELD_VIRIAL) HIPCHK(hipMemcpy(virial, fs.vir, fsz * sizeof(double), hipMemcpyDeviceToHost));
-    if (fields & PGT_FIELD_KINETIC) HIPCHK(hipMemcpy(kinetic, fs.kin, fsz * sizeof(double), hipMemcpyDeviceToHost));
-    if (fields & PGT_FIELD_ELASTIC) HIPCHK(hipMemcpy(elastic, fs.ela, fsz * sizeof(double), hipMemcpyDeviceToHost));
+    // through pinned staging memory
+    if ((rc = e->h_stage.ensure(3 * fsz * sizeof(double)))) return rc;
+    double *h = reinterpret_cast<double *>(e->h_stage.p);
+    const unsigned bits[3] = {PGT_FIELD_VIRIAL, PGT_FIELD_KINETIC, PGT_FIELD_ELASTIC};
+    const double *dsrc[3] = {fs.vir, fs.kin, fs.ela};
+    double *hdst[3] = {virial, kinetic, elastic};
+    for (int k = 0; k < 3; ++k)
+        if (fields & bits[k])
+            HIPCHK(hipMemcpyAsync(h + k * fsz, dsrc[k], fsz * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int k = 0; k < 3; ++k)
+        if (fields & bits[k]) std::memcpy(hdst[k], h + k * fsz, fsz * sizeof(double));
     return PGT_OK;
 }
 

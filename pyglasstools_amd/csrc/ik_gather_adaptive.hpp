@@ -130,7 +130,7 @@ __global__ __launch_bounds__(128) void k_gather_adaptive(Geom g, int n, int G, c
         if (ELA) store(out.ela, ae, 1.0);
         if (KIN) store(out.kin, ak, cgnorm);
     }
-    if (hit_cap) *capped = 1;
+    if (hit_cap) atomicOr(capped, 1);
     nball = wave_sum(nball);
     ncontrib = wave_sum(ncontrib);
     if ((lane & 63) == 0) {

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ik_gather.hpp"
+#include "ik_gather_tile.hpp"
 #include "ik_launch.hpp"
 #include "pgt_ik.h"
 
@@ -56,11 +57,43 @@ void launch_gather_f(const LaunchCtx &c, unsigned fields)
     else launch_gather_b<D, OD, CG, NQ, false, false, ER>(c);
 }
 
+template <int D, int OD, bool VIR, bool ELA, bool KIN, typename ER>
+void launch_tile_t(const LaunchCtx &c)
+{
+    using CP = TileCaps<D, OD, ELA, ER>;
+    const dim3 grid((unsigned)((c.G + CP::TP - 1) / CP::TP), (unsigned)c.nf);
+    c.pl_launched = 1;
+    hipLaunchKernelGGL((k_gather_tile<D, OD, 0, 5, VIR, ELA, KIN, ER>), grid, dim3(CP::BLK), 0, c.stream, c.g, c.n, c.G,
+                       c.grid, c.perm, c.R, c.gld, c.cgnorm, c.cstart, c.spos, c.skin, c.cap, c.bcount,
+                       (const ER *)c.bonds, c.out, c.flags);
+}
+
+template <int D, int OD, bool VIR, bool ELA, typename ER>
+void launch_tile_k(const LaunchCtx &c)
+{
+    if (c.skin) launch_tile_t<D, OD, VIR, ELA, true, ER>(c);
+    else launch_tile_t<D, OD, VIR, ELA, false, ER>(c);
+}
+
+template <int D, int OD, typename ER>
+void launch_tile_f(const LaunchCtx &c, unsigned fields)
+{
+    const bool V = fields & PGT_FIELD_VIRIAL, E = fields & PGT_FIELD_ELASTIC;
+    if (V && E) launch_tile_k<D, OD, true, true, ER>(c);
+    else if (V) launch_tile_k<D, OD, true, false, ER>(c);
+    else if (E) launch_tile_k<D, OD, false, true, ER>(c);
+    else launch_tile_k<D, OD, false, false, ER>(c);
+}
+
 // The Octic kernel gets the unrolled 5-node rule (the driver scripts' order)
-// and the FP32 form; Rect and Mollifier run the generic FP64 rule.
+// and the FP32 form, by the tile gather when c.tile is set; Rect and
+// Mollifier run the generic FP64 rule.
 template <int D, int OD, typename ER>
 void launch_gather_c(const LaunchCtx &c, unsigned fields, int cg)
 {
+    if constexpr (!(D == 3 && OD == 2)) {
+        if (c.tile && cg == 0 && c.R.n == 5) return launch_tile_f<D, OD, ER>(c, fields);
+    }
     if (cg == 0) {
         if (c.R.n == 5 && !(D == 3 && OD == 2)) launch_gather_f<D, OD, 0, 5, ER>(c, fields);
         else launch_gather_f<D, OD, 0, 0, ER>(c, fields);

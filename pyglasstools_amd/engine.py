@@ -17,13 +17,14 @@ POT = {"poly12": 0, "poly18": 1, "poly10": 2, "poly106": 3, "polylj": 4}
 CG = {"octic": 0, "rect": 1, "mollifier": 2}
 FIELD_VIRIAL, FIELD_KINETIC, FIELD_ELASTIC = 1, 2, 4
 PREC = {"fp64": 0, "fp32": 1}
+MLP_IN_PAD = 8  # floats per network input row (kMlpInPad)
 
 # every symbol include/pgt_ik.h declares
 EXPORTED = [
     "pgt_last_error", "pgt_device_count", "pgt_ik_create", "pgt_ik_destroy", "pgt_ik_set_box",
     "pgt_ik_set_particles", "pgt_ik_set_grid", "pgt_ik_compute", "pgt_ik_compute_frames", "pgt_ik_enqueue_frames",
-    "pgt_ik_finish", "pgt_ik_set_stream",
-    "pgt_ik_set_tuning", "pgt_ik_get_stats", "pgt_eval_pair_force", "pgt_eval_cgfunc",
+    "pgt_ik_finish", "pgt_ik_release_enqueued", "pgt_ik_set_stream",
+    "pgt_ik_set_tuning", "pgt_ik_set_gather_mode", "pgt_ik_get_stats", "pgt_eval_pair_force", "pgt_eval_cgfunc",
     "pgt_mlp_nparam", "pgt_mlp_work_floats", "pgt_mlp_forward", "pgt_mlp_grad", "pgt_mlp_adam", "pgt_mlp_infer",
     "pgt_ik_bond_samples", "pgt_ik_compute_surrogate",
 ]
@@ -89,8 +90,10 @@ def lib():
         L.pgt_ik_compute_frames.argtypes = [vp, i, i64, vp, vp, vp, vp, vp, ctypes.c_uint, vp, vp, vp]
         L.pgt_ik_enqueue_frames.argtypes = [vp, i, i64, vp, vp, vp, vp, vp, ctypes.c_uint, vp, vp, vp]
         L.pgt_ik_finish.argtypes = [vp]
+        L.pgt_ik_release_enqueued.argtypes = [vp]
         L.pgt_ik_set_stream.argtypes = [vp, vp]
         L.pgt_ik_set_tuning.argtypes = [vp, i, i, i, i]
+        L.pgt_ik_set_gather_mode.argtypes = [vp, i]
         L.pgt_ik_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.pgt_eval_pair_force.argtypes = [i, vp, i, d, i64, vp, vp, vp, vp, vp, vp]
         L.pgt_eval_cgfunc.argtypes = [i, d, i, i, i, d, d, i, i64, vp, vp, vp, vp, vp]
@@ -130,6 +133,29 @@ def _ptr(a):
     if hasattr(a, "data_ptr"):  # torch tensor on the device
         return ctypes.c_void_p(a.data_ptr())
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _dev(a, dtype, numel, name):
+    """Device pointer of a torch tensor after checking what the kernels
+    assume: on the GPU, contiguous, of `dtype`, holding at least `numel`
+    elements.  None passes through.  Anything else raises ValueError before
+    any launch: the C side cannot check sizes, and a host, strided, narrower
+    or shorter buffer would be read or written out of bounds on the device."""
+    if a is None:
+        return None
+    import torch
+
+    if not isinstance(a, torch.Tensor):
+        raise ValueError("%s must be a torch tensor on the GPU (got %s)" % (name, type(a).__name__))
+    if not a.is_cuda:
+        raise ValueError("%s must be on the GPU (got device %s)" % (name, a.device))
+    if a.dtype != dtype:
+        raise ValueError("%s must be %s (got %s)" % (name, dtype, a.dtype))
+    if not a.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+    if a.numel() < numel:
+        raise ValueError("%s holds %d elements; this call reads or writes %d" % (name, a.numel(), numel))
+    return ctypes.c_void_p(a.data_ptr())
 
 
 def _f64(a, shape=None):
@@ -225,16 +251,24 @@ class Engine(object):
         """Training samples of the current frame and grid (device float32
         tensors: d_x rows of 8, d_t the 5-node integrals); returns the pair
         count (only counts when d_x is None)."""
+        import torch
+
         n = ctypes.c_int64(0)
-        _check(lib().pgt_ik_bond_samples(self.h, max_pairs, _ptr(d_x), _ptr(d_t), ctypes.byref(n)),
+        max_pairs = int(max_pairs)
+        px = _dev(d_x, torch.float32, MLP_IN_PAD * max_pairs, "d_x")
+        pt = _dev(d_t, torch.float32, max_pairs, "d_t")
+        _check(lib().pgt_ik_bond_samples(self.h, max_pairs, px, pt, ctypes.byref(n)),
                "pgt_ik_bond_samples")
         return n.value
 
     def compute_surrogate(self, d_params, bound_inside, bound_crossing, tol):
         """Virial field (G, 2, 2) with the network's bond integral where the
         pair class's error bound is within tol, the 5-node rule elsewhere."""
+        import torch
+
         out = np.zeros((self.G, 2, 2))
-        _check(lib().pgt_ik_compute_surrogate(self.h, _ptr(d_params), bound_inside, bound_crossing, tol, _ptr(out)),
+        pp = _dev(d_params, torch.float32, mlp_nparam(), "d_params")
+        _check(lib().pgt_ik_compute_surrogate(self.h, pp, bound_inside, bound_crossing, tol, _ptr(out)),
                "pgt_ik_compute_surrogate")
         return out
 
@@ -251,13 +285,37 @@ class Engine(object):
         _check(lib().pgt_ik_compute(self.h, mask, *ptrs), "pgt_ik_compute")
         return out
 
-    def compute_frames(self, nframes, n, d_pos, d_diam, d_mass=None, d_vel=None, d_disp=None, fields=("virial",),
-                       d_virial=None, d_kinetic=None, d_elastic=None):
+    def _frame_args(self, nframes, n, d_pos, d_diam, d_mass, d_vel, d_disp, fields, d_virial, d_kinetic, d_elastic):
+        """Field mask and checked device pointers of a batched call."""
+        import torch
+
+        nframes, n = int(nframes), int(n)
+        if nframes < 0 or n < 0:
+            raise ValueError("frame and particle counts must be non-negative")
         mask = (FIELD_VIRIAL if "virial" in fields else 0) | (FIELD_KINETIC if "kinetic" in fields else 0) | \
                (FIELD_ELASTIC if "elastic" in fields else 0)
-        _check(lib().pgt_ik_compute_frames(self.h, nframes, n, _ptr(d_pos), _ptr(d_diam), _ptr(d_mass), _ptr(d_vel),
-                                           _ptr(d_disp), mask, _ptr(d_virial), _ptr(d_kinetic), _ptr(d_elastic)),
-               "pgt_ik_compute_frames")
+        f64 = torch.float64
+        m = nframes * n
+        fsz = nframes * self.G * self.dim * self.dim
+        if d_pos is None or d_diam is None:
+            raise ValueError("positions and diameters are required")
+        ptrs = [_dev(d_pos, f64, 3 * m, "d_pos"), _dev(d_diam, f64, m, "d_diam"), _dev(d_mass, f64, m, "d_mass"),
+                _dev(d_vel, f64, 3 * m, "d_vel"), _dev(d_disp, f64, 3 * m, "d_disp")]
+        outs = []
+        for name, bit, a in (("d_virial", FIELD_VIRIAL, d_virial), ("d_kinetic", FIELD_KINETIC, d_kinetic),
+                             ("d_elastic", FIELD_ELASTIC, d_elastic)):
+            if mask & bit and a is None:
+                raise ValueError("%s is required for the selected fields" % name)
+            outs.append(_dev(a, f64, fsz, name) if mask & bit else None)
+        return nframes, n, mask, ptrs, outs
+
+    def compute_frames(self, nframes, n, d_pos, d_diam, d_mass=None, d_vel=None, d_disp=None, fields=("virial",),
+                       d_virial=None, d_kinetic=None, d_elastic=None):
+        """Fields of nframes device-resident frames (torch float64 tensors
+        on the GPU, frames back to back); outputs nframes x G x D x D."""
+        nframes, n, mask, ptrs, outs = self._frame_args(nframes, n, d_pos, d_diam, d_mass, d_vel, d_disp, fields,
+                                                        d_virial, d_kinetic, d_elastic)
+        _check(lib().pgt_ik_compute_frames(self.h, nframes, n, *ptrs, mask, *outs), "pgt_ik_compute_frames")
 
     def set_stream(self, stream_handle):
         _check(lib().pgt_ik_set_stream(self.h, ctypes.c_void_p(stream_handle) if stream_handle else None),
@@ -268,18 +326,26 @@ class Engine(object):
         """compute_frames without host synchronisation or allocation (graph
         capture); needs an earlier compute_frames of the same shape; check
         with finish()."""
-        mask = (FIELD_VIRIAL if "virial" in fields else 0) | (FIELD_KINETIC if "kinetic" in fields else 0) | \
-               (FIELD_ELASTIC if "elastic" in fields else 0)
-        _check(lib().pgt_ik_enqueue_frames(self.h, nframes, n, _ptr(d_pos), _ptr(d_diam), _ptr(d_mass), _ptr(d_vel),
-                                           _ptr(d_disp), mask, _ptr(d_virial), _ptr(d_kinetic), _ptr(d_elastic)),
-               "pgt_ik_enqueue_frames")
+        nframes, n, mask, ptrs, outs = self._frame_args(nframes, n, d_pos, d_diam, d_mass, d_vel, d_disp, fields,
+                                                        d_virial, d_kinetic, d_elastic)
+        _check(lib().pgt_ik_enqueue_frames(self.h, nframes, n, *ptrs, mask, *outs), "pgt_ik_enqueue_frames")
 
     def finish(self):
         _check(lib().pgt_ik_finish(self.h), "pgt_ik_finish")
 
+    def release_enqueued(self):
+        """No captured graph replays this engine's enqueued launches any
+        more: calls may regrow its buffers again."""
+        _check(lib().pgt_ik_release_enqueued(self.h), "pgt_ik_release_enqueued")
+
     def set_tuning(self, threads_per_block=0, frames_per_launch=0, points_per_lane=0, cell_reach=0):
         _check(lib().pgt_ik_set_tuning(self.h, threads_per_block, frames_per_launch, points_per_lane, cell_reach),
                "pgt_ik_set_tuning")
+
+    def set_gather_mode(self, mode):
+        """'tile' (LDS-staged workgroup tiles) or 'lane' (one lane per grid
+        point from global memory) for the Octic 5-node rule."""
+        _check(lib().pgt_ik_set_gather_mode(self.h, {"lane": 0, "tile": 1}[mode]), "pgt_ik_set_gather_mode")
 
     def stats(self):
         s = Stats()
@@ -335,19 +401,50 @@ def mlp_work_floats(n):
     return int(lib().pgt_mlp_work_floats(n))
 
 
+def _rows(x):
+    """Sample count of a network input (rows of MLP_IN_PAD floats)."""
+    if getattr(x, "dim", lambda: 0)() != 2 or x.shape[1] != MLP_IN_PAD:
+        raise ValueError("network inputs are rows of %d floats (got shape %s)" % (MLP_IN_PAD, tuple(x.shape)))
+    return int(x.shape[0])
+
+
+def _net(params):
+    import torch
+
+    return _dev(params, torch.float32, mlp_nparam(), "params")
+
+
 def mlp_forward(params, x, y, work):
-    _check(lib().pgt_mlp_forward(_ptr(params), x.shape[0], _ptr(x), _ptr(y), _ptr(work), _stream()),
-           "pgt_mlp_forward")
+    import torch
+
+    n = _rows(x)
+    f = torch.float32
+    _check(lib().pgt_mlp_forward(_net(params), n, _dev(x, f, n * MLP_IN_PAD, "x"), _dev(y, f, n, "y"),
+                                 _dev(work, f, mlp_work_floats(n), "work"), _stream()), "pgt_mlp_forward")
 
 
 def mlp_grad(params, x, t, grad, loss, work):
-    _check(lib().pgt_mlp_grad(_ptr(params), x.shape[0], _ptr(x), _ptr(t), _ptr(grad), _ptr(loss), _ptr(work),
-                              _stream()), "pgt_mlp_grad")
+    import torch
+
+    n = _rows(x)
+    f = torch.float32
+    _check(lib().pgt_mlp_grad(_net(params), n, _dev(x, f, n * MLP_IN_PAD, "x"), _dev(t, f, n, "t"),
+                              _dev(grad, f, mlp_nparam(), "grad"), _dev(loss, f, 1, "loss"),
+                              _dev(work, f, mlp_work_floats(n), "work"), _stream()), "pgt_mlp_grad")
 
 
 def mlp_adam(params, m, v, grad, step, lr):
-    _check(lib().pgt_mlp_adam(_ptr(params), _ptr(m), _ptr(v), _ptr(grad), step, lr, _stream()), "pgt_mlp_adam")
+    import torch
+
+    f, k = torch.float32, mlp_nparam()
+    _check(lib().pgt_mlp_adam(_net(params), _dev(m, f, k, "m"), _dev(v, f, k, "v"), _dev(grad, f, k, "grad"), step,
+                              lr, _stream()), "pgt_mlp_adam")
 
 
 def mlp_infer(params, x, y):
-    _check(lib().pgt_mlp_infer(_ptr(params), x.shape[0], _ptr(x), _ptr(y), _stream()), "pgt_mlp_infer")
+    import torch
+
+    n = _rows(x)
+    f = torch.float32
+    _check(lib().pgt_mlp_infer(_net(params), n, _dev(x, f, n * MLP_IN_PAD, "x"), _dev(y, f, n, "y"), _stream()),
+           "pgt_mlp_infer")

@@ -1,0 +1,238 @@
+"""The benchmark workloads at their own sizes, against the FP64 oracle,
+through the C ABI:
+
+  2D N=65,536, 100 frames, 256 x 256 grid, FP64 virial + kinetic: frames 0,
+    50 and 99 on every grid point at the drop-in tolerance, contribution and
+    ball counts of all 100 frames exactly;
+  3D N=131,072, 20 frames, 64^3 grid, FP32 form: frames 0 and 19 on 2,048
+    sampled points, each component within 1e-5 of its largest |T|; the FP64
+    form on the same points at the drop-in tolerance with exact counts; and
+    over all 20 frames the FP32 job's ball count equal to the FP64 job's and
+    its contribution count within 1e-6 of it (pairs at the cutoff);
+  8 systems of uneven N (4k .. 96k), 128 x 128 grid each, two frames per
+    system, captured into one HIP graph on streams forked inside it and
+    replayed: every system's frames against the oracle, FP64 and FP32.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from pyglasstools_amd import engine, workload
+
+from conftest import scaled_error
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+TOL_FP32 = 1e-5
+MAX_RCUT = max(1.25 * 1.62, 1.62)  # Calculator.h:41-45 at the workloads' largest diameter bound
+
+
+def _stack(frames, key, dev):
+    import torch
+
+    return torch.tensor(np.stack([f[key] for f in frames]), dtype=torch.float64, device=dev)
+
+
+def assert_fp64_parity(got, ref, absum, where):
+    """The drop-in bar |T_gpu - T_oracle| <= 1e-10 max(|T_oracle|, floor S),
+    S = sum of |contributions|, at every point with floor 1e-4, the FP64
+    bound for summing up to ~90 terms in another order (90 x 2^-53 S ~ 1e-14
+    S, where the stricter floor 1e-6 asks for ~1 ulp of S); the points that
+    meet only the looser floor (cancellation to below 1e-4 of S) are counted
+    and must stay a vanishing fraction."""
+    strict = scaled_error(got, ref, absum, floor=1e-6)
+    loose = scaled_error(got, ref, absum, floor=1e-4)
+    npts = len(ref)
+    n_strict = int((strict.reshape(npts, -1) > TOL).any(axis=1).sum())
+    print("%s: max scaled err %.3e (floor 1e-6), %.3e (floor 1e-4); %d of %d grid points over the 1e-6 floor bar"
+          % (where, strict.max(), loose.max(), n_strict, npts), flush=True)
+    assert loose.max() <= TOL
+    assert n_strict <= max(1, npts // 10000)
+    return strict.max()
+
+
+def _fp32_rel(got, ref):
+    """max over grid points of |got - ref| per component / max |ref| of that component."""
+    D = ref.shape[-1]
+    worst = 0.0
+    for a in range(D):
+        for b in range(a, D):
+            scale = np.abs(ref[:, b, a]).max()
+            worst = max(worst, float(np.abs(got[:, b, a] - ref[:, b, a]).max() / max(scale, 1e-300)))
+    return worst
+
+
+def test_c2_fp64_100_frames(gpu):
+    import torch
+
+    dev = torch.device("cuda:0")
+    F, N, NG = 100, 65536, 256
+    print("c2: generating %d frames of N=%d" % (F, N), flush=True)
+    frames = [workload.make_system(N, 2, 2000 + f) for f in range(F)]
+    L = frames[0]["L"]
+    grid = workload.cartesian_grid_2d(L[0], NG)
+    G = len(grid)
+    d0 = float(frames[0]["diam"].max())
+    max_rcut = max(1.25 * d0, d0)
+    e = engine.Engine(dim=2, max_rcut=max_rcut)
+    e.set_box(L, [1, 1, 0])
+    e.set_grid(grid)
+    e.set_tuning(threads_per_block=128, frames_per_launch=F, points_per_lane=2, cell_reach=1)
+    pos, diam, mass, vel = (_stack(frames, k, dev) for k in ("pos", "diam", "mass", "vel"))
+    ov = torch.full((F, G, 2, 2), np.nan, dtype=torch.float64, device=dev)
+    ok = torch.full_like(ov, np.nan)
+    print("c2: one 100-frame job", flush=True)
+    e.compute_frames(F, N, pos, diam, mass, vel, None, fields=("virial", "kinetic"), d_virial=ov, d_kinetic=ok)
+    torch.cuda.synchronize()
+    st = e.stats()
+    n_nonzero = n_ball = 0
+    for f in range(F):
+        ref = oracle.ik_compute(frames[f], grid, fields=("virial", "kinetic"), max_rcut=max_rcut,
+                                with_abs=f in (0, 50, 99))
+        n_nonzero += ref["n_nonzero"]
+        n_ball += ref["n_ball"]
+        if f in (0, 50, 99):
+            gv, gk = ov[f].cpu().numpy(), ok[f].cpu().numpy()
+            assert_fp64_parity(gv, ref["virial"], ref["virial_abs"], "c2 frame %d virial" % f)
+            ek = np.abs(gk - ref["kinetic"]).max() / np.abs(ref["kinetic"]).max()
+            print("c2 frame %d: kinetic %.3e" % (f, ek), flush=True)
+            assert ek <= TOL
+    print("c2: contributions %d (oracle %d), balls %d (oracle %d)" % (st["n_contrib"], n_nonzero, st["n_ball"],
+                                                                        n_ball), flush=True)
+    assert st["n_contrib"] == n_nonzero
+    assert st["n_ball"] == n_ball
+
+
+def test_c3_fp32_form_20_frames(gpu):
+    import torch
+
+    dev = torch.device("cuda:0")
+    F, N, NG = 20, 131072, 64
+    print("c3: generating %d 3D frames of N=%d" % (F, N), flush=True)
+    frames = [workload.make_system(N, 3, 3000 + f) for f in range(F)]
+    L = frames[0]["L"]
+    grid = workload.cartesian_grid_3d(L[0], NG)
+    G = len(grid)
+    d0 = float(frames[0]["diam"].max())
+    max_rcut = max(1.25 * d0, d0)
+    pos, diam = _stack(frames, "pos", dev), _stack(frames, "diam", dev)
+    out = {}
+    counts = {}
+    for prec in ("fp32", "fp64"):
+        print("c3: %s 20-frame job" % prec, flush=True)
+        e = engine.Engine(dim=3, max_rcut=max_rcut, precision=prec)
+        e.set_box(L, [1, 1, 1])
+        e.set_grid(grid)
+        e.set_tuning(frames_per_launch=F)
+        ov = torch.full((F, G, 3, 3), np.nan, dtype=torch.float64, device=dev)
+        e.compute_frames(F, N, pos, diam, None, None, None, fields=("virial",), d_virial=ov)
+        torch.cuda.synchronize()
+        out[prec] = ov
+        counts[prec] = (e.stats()["n_contrib"], e.stats()["n_ball"])
+        e.close()
+    print("c3: counts fp32 %s fp64 %s" % (counts["fp32"], counts["fp64"]), flush=True)
+    # the same particles in every ball; the FP32 form decides the pair cutoff
+    # in FP32, so pairs within FP32 rounding of it (where the force is ~0) may
+    # count on one side only
+    assert counts["fp32"][1] == counts["fp64"][1]
+    assert abs(counts["fp32"][0] - counts["fp64"][0]) <= 1e-6 * counts["fp64"][0]
+    idx = np.sort(np.random.default_rng(7).choice(G, size=2048, replace=False))
+    for f in (0, F - 1):
+        ref = oracle.ik_compute(frames[f], grid[idx], fields=("virial",), max_rcut=max_rcut, with_abs=True)
+        v32 = out["fp32"][f].cpu().numpy()[idx]
+        v64 = out["fp64"][f].cpu().numpy()[idx]
+        e32 = _fp32_rel(v32, ref["virial"])
+        print("c3 frame %d: fp32 worst component err %.3e" % (f, e32), flush=True)
+        assert e32 <= TOL_FP32
+        assert_fp64_parity(v64, ref["virial"], ref["virial_abs"], "c3 frame %d fp64 form" % f)
+        # exact counts on the sampled points: one FP64 frame with the sample as its grid
+        e = engine.Engine(dim=3, max_rcut=max_rcut)
+        e.set_system(frames[f], grid[idx])
+        got = e.compute(fields=("virial",))["virial"]
+        st = e.stats()
+        assert_fp64_parity(got, ref["virial"], ref["virial_abs"], "c3 frame %d sampled grid" % f)
+        assert st["n_contrib"] == ref["n_nonzero"] and st["n_ball"] == ref["n_ball"]
+        e.close()
+
+
+C5_N = (4096, 8192, 16384, 24576, 32768, 49152, 65536, 98304)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_c5_graph_sweep_matches_oracle(gpu, precision):
+    """The sweep's graph (every system's enqueue_frames captured, systems on 4
+    streams forked from the capture stream and joined back), replayed twice;
+    each system's frames against the oracle on its own grid."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    FR, NG, K = 2, 128, 4
+    jobs = []
+    for k, N in enumerate(C5_N):
+        frames = [workload.make_system(N, 2, 5000 + 1000 * k + f) for f in range(FR)]
+        L = frames[0]["L"]
+        d0 = float(frames[0]["diam"].max())
+        max_rcut = max(1.25 * d0, d0)
+        grid = workload.cartesian_grid_2d(L[0], NG)
+        e = engine.Engine(dim=2, max_rcut=max_rcut, precision=precision)
+        e.set_box(L, [1, 1, 0])
+        e.set_grid(grid)
+        e.set_tuning(frames_per_launch=FR)
+        j = dict(e=e, N=N, frames=frames, grid=grid, max_rcut=max_rcut,
+                 pos=_stack(frames, "pos", dev), diam=_stack(frames, "diam", dev),
+                 mass=_stack(frames, "mass", dev), vel=_stack(frames, "vel", dev))
+        j["ov"] = torch.zeros((FR, len(grid), 2, 2), dtype=torch.float64, device=dev)
+        j["ok"] = torch.zeros_like(j["ov"])
+        jobs.append(j)
+    s = torch.cuda.Stream()
+    side = [torch.cuda.Stream() for _ in range(K)]
+    order = sorted(range(len(jobs)), key=lambda i: jobs[i]["N"])
+    lane = {order[k]: (k if k < K else 2 * K - 1 - k) % K for k in range(len(order))}
+
+    def args(j):
+        return (FR, j["N"], j["pos"], j["diam"], j["mass"], j["vel"], None)
+
+    for i, j in enumerate(jobs):
+        j["e"].set_stream(side[lane[i]].cuda_stream)
+        with torch.cuda.stream(side[lane[i]]):
+            j["e"].compute_frames(*args(j), fields=("virial", "kinetic"), d_virial=j["ov"], d_kinetic=j["ok"])
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        for st_ in side:
+            st_.wait_stream(s)
+        for j in jobs:
+            j["e"].enqueue_frames(*args(j), fields=("virial", "kinetic"), d_virial=j["ov"], d_kinetic=j["ok"])
+        for st_ in side:
+            s.wait_stream(st_)
+    for rep in range(2):
+        for j in jobs:
+            j["ov"].fill_(np.nan)
+            j["ok"].fill_(np.nan)
+        torch.cuda.synchronize()
+        print("c5 %s: replay %d" % (precision, rep), flush=True)
+        g.replay()
+        torch.cuda.synchronize()
+        for j in jobs:
+            j["e"].finish()
+    for j in jobs:
+        st = j["e"].stats()  # counts of the last replay
+        n_nonzero = n_ball = 0
+        for f in range(FR):
+            ref = oracle.ik_compute(j["frames"][f], j["grid"], fields=("virial", "kinetic"), max_rcut=j["max_rcut"],
+                                    with_abs=True)
+            n_nonzero += ref["n_nonzero"]
+            n_ball += ref["n_ball"]
+            gv, gk = j["ov"][f].cpu().numpy(), j["ok"][f].cpu().numpy()
+            ek = np.abs(gk - ref["kinetic"]).max() / np.abs(ref["kinetic"]).max()
+            if precision == "fp64":
+                ev = assert_fp64_parity(gv, ref["virial"], ref["virial_abs"], "c5 N=%d frame %d" % (j["N"], f))
+                ok = True
+            else:
+                ev = _fp32_rel(gv, ref["virial"])
+                ok = ev <= TOL_FP32
+            print("c5 %s N=%d frame %d: virial err %.3e, kinetic %.3e" % (precision, j["N"], f, ev, ek), flush=True)
+            assert ok and ek <= TOL
+        assert st["n_contrib"] == n_nonzero and st["n_ball"] == n_ball
+        j["e"].set_stream(0)
+        j["e"].release_enqueued()

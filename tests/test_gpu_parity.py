@@ -711,3 +711,44 @@ def test_graph_captured_batches_match_eager(gpu, forked):
         del junk
     for j in jobs:
         j["e"].set_stream(0)
+
+
+def test_enqueued_buffers_are_held_until_released(gpu):
+    """After enqueue_frames the engine refuses calls that would free, grow or
+    replace the buffers a captured graph replays (set_grid, a larger
+    compute_frames), with PGT_ERR_STATE, until release_enqueued; a job of the
+    same shape still runs.  Host-side checks only: nothing faulting is run."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    N, nf = 2000, 2
+    frames = [workload.make_system(N, 2, 970 + f) for f in range(nf + 2)]
+    L = frames[0]["L"]
+    grid = workload.cartesian_grid_2d(L[0], 16)
+    e = engine.Engine(dim=2, max_rcut=max(1.25 * 1.62, 1.62))
+    e.set_box(L, [1, 1, 0])
+    e.set_grid(grid)
+    e.set_tuning(frames_per_launch=nf)
+    t = lambda k, m: torch.tensor(np.stack([f[k] for f in frames[:m]]), dtype=torch.float64, device=dev)
+    pos, diam = t("pos", nf), t("diam", nf)
+    ov = torch.zeros((nf, len(grid), 2, 2), dtype=torch.float64, device=dev)
+    e.compute_frames(nf, N, pos, diam, d_virial=ov)
+    e.enqueue_frames(nf, N, pos, diam, d_virial=ov)
+    e.finish()
+    with pytest.raises(engine.PGTError, match="PGT_ERR_STATE"):
+        e.set_grid(grid)
+    # more frames per launch needs larger buffers: refused
+    e.set_tuning(frames_per_launch=nf + 2)
+    pos4, diam4 = t("pos", nf + 2), t("diam", nf + 2)
+    ov4 = torch.zeros((nf + 2, len(grid), 2, 2), dtype=torch.float64, device=dev)
+    with pytest.raises(engine.PGTError, match="PGT_ERR_STATE"):
+        e.compute_frames(nf + 2, N, pos4, diam4, d_virial=ov4)
+    # the held shape still runs
+    e.set_tuning(frames_per_launch=nf)
+    e.compute_frames(nf, N, pos, diam, d_virial=ov)
+    e.release_enqueued()
+    e.set_grid(grid)
+    e.set_tuning(frames_per_launch=nf + 2)
+    e.compute_frames(nf + 2, N, pos4, diam4, d_virial=ov4)
+    torch.cuda.synchronize()
+    assert torch.equal(ov4[:nf], ov)

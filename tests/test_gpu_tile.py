@@ -1,0 +1,192 @@
+"""The LDS-staged tile gather (pgt_ik_set_gather_mode 1) against the oracle
+and against the lane gather, through the C ABI: 2D and 3D, FP64 and the FP32
+form, virial / kinetic / elastic fields, the fixture trajectory, windows that
+wrap a whole periodic axis, 3D windows processed in several LDS chunks, a
+non-periodic axis, cell reach 2, and several frames per launch."""
+import numpy as np
+import pytest
+
+import oracle
+from pyglasstools_amd import engine, workload
+from pyglasstools_amd.gsd_reader import open as gsd_open
+
+from conftest import FIXTURE_GSD, scaled_error
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _engine(dim, max_rcut, precision="fp64", mode="tile", reach=None):
+    e = engine.Engine(dim=dim, max_rcut=max_rcut, precision=precision)
+    e.set_gather_mode(mode)
+    if reach:
+        e.set_tuning(cell_reach=reach)
+    return e
+
+
+def _check(got, ref, fields, st=None, tol=TOL):
+    for name in fields:
+        if name == "kinetic":
+            err = np.abs(got[name] - ref[name]).max() / max(np.abs(ref[name]).max(), 1e-300)
+        else:
+            err = scaled_error(got[name], ref[name], ref["virial_abs"]).max()
+        print("  %s: max scaled err %.3e" % (name, err), flush=True)
+        assert err <= tol, (name, err)
+    if st is not None:
+        assert st["n_ball"] == ref["n_ball"]
+        if "virial" in fields:
+            assert st["n_contrib"] == ref["n_nonzero"]
+
+
+@pytest.mark.parametrize("N,ng,reach", [(2000, 32, 1), (2000, 32, 2), (500, 40, 1), (9000, 96, 1)])
+def test_tile_2d_fp64_virial_kinetic(gpu, N, ng, reach):
+    sysd = workload.make_system(N, 2, 8100 + N)
+    grid = workload.cartesian_grid_2d(sysd["L"][0], ng)
+    mr = max(1.25 * float(sysd["diam"].max()), float(sysd["diam"].max()))
+    print("tile 2d N=%d grid %d^2 reach %d" % (N, ng, reach), flush=True)
+    e = _engine(2, mr, reach=reach)
+    e.set_system(sysd, grid)
+    got = e.compute(fields=("virial", "kinetic"))
+    ref = oracle.ik_compute(sysd, grid, fields=("virial", "kinetic"), max_rcut=mr, with_abs=True)
+    _check(got, ref, ("virial", "kinetic"), e.stats())
+
+
+def test_tile_fixture_frames_match_oracle(gpu):
+    with gsd_open(FIXTURE_GSD) as t:
+        for f in (180, 199):
+            snap = t[f]
+            L = np.array([snap.configuration.box[0], snap.configuration.box[1], 1.0])
+            sysd = {"dim": 2, "L": L, "periodic": np.array([1, 1, 0]), "pos": snap.particles.position.astype(np.float64),
+                    "diam": snap.particles.diameter.astype(np.float64), "mass": np.ones(snap.particles.N),
+                    "vel": np.zeros((snap.particles.N, 3))}
+            n = int(L[0] / 0.25)
+            p = np.linspace(-L[0] / 2, L[0] / 2, n)
+            grid = np.zeros((n * n, 3))
+            grid[:, 0] = np.repeat(p, n)
+            grid[:, 1] = np.tile(p, n)
+            mr = max(1.25 * float(sysd["diam"].max()), float(sysd["diam"].max()))
+            print("tile fixture frame %d: N=%d, %d grid points" % (f, snap.particles.N, len(grid)), flush=True)
+            e = _engine(2, mr)
+            e.set_system(sysd, grid)
+            got = e.compute(fields=("virial",))
+            ref = oracle.ik_compute(sysd, grid, fields=("virial",), max_rcut=mr, with_abs=True)
+            _check(got, ref, ("virial",), e.stats())
+
+
+def test_tile_elastic_and_fp32_2d(gpu):
+    sysd = workload.make_system(3000, 2, 8300)
+    rng = np.random.default_rng(5)
+    sysd["disp"] = rng.normal(0.0, 0.03, (3000, 3)) * np.array([1, 1, 0])
+    grid = workload.cartesian_grid_2d(sysd["L"][0], 40)
+    mr = max(1.25 * float(sysd["diam"].max()), float(sysd["diam"].max()))
+    print("tile 2d elastic", flush=True)
+    e = _engine(2, mr)
+    e.set_system(sysd, grid)
+    got = e.compute(fields=("virial", "kinetic", "elastic"))
+    ref = oracle.ik_compute(sysd, grid, fields=("virial", "kinetic", "elastic"), max_rcut=mr, with_abs=True)
+    _check(got, ref, ("virial", "kinetic", "elastic"))
+    print("tile 2d fp32 form vs lane fp32 form", flush=True)
+    outs = {}
+    for mode in ("tile", "lane"):
+        e = _engine(2, mr, precision="fp32", mode=mode)
+        e.set_system(sysd, grid)
+        outs[mode] = (e.compute(fields=("virial",))["virial"], e.stats())
+    rel = np.abs(outs["tile"][0] - ref["virial"]).max() / np.abs(ref["virial"]).max()
+    print("  fp32 tile err vs oracle %.3e" % rel, flush=True)
+    assert rel <= 1e-5
+    # both FP32 forms: the same FP32 terms up to the compiler's contraction
+    # choices, summed in FP64 in different orders
+    assert np.abs(outs["tile"][0] - outs["lane"][0]).max() <= 1e-6 * np.abs(ref["virial"]).max()
+    assert outs["tile"][1]["n_contrib"] == outs["lane"][1]["n_contrib"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_tile_3d_several_chunks(gpu, precision):
+    """3D windows hold more particles than one LDS chunk (384), so every tile
+    stages its window in pieces."""
+    sysd = workload.make_system(6000, 3, 8400)
+    grid = workload.cartesian_grid_3d(sysd["L"][0], 16)
+    mr = max(1.25 * float(sysd["diam"].max()), float(sysd["diam"].max()))
+    print("tile 3d %s N=6000 grid 16^3" % precision, flush=True)
+    e = _engine(3, mr, precision=precision)
+    e.set_system(sysd, grid)
+    got = e.compute(fields=("virial",))
+    st = e.stats()
+    ref = oracle.ik_compute(sysd, grid, fields=("virial",), max_rcut=mr, with_abs=True)
+    if precision == "fp64":
+        _check(got, ref, ("virial",), st)
+    else:
+        worst = 0.0
+        for a in range(3):
+            for b in range(a, 3):
+                worst = max(worst, np.abs(got["virial"][:, b, a] - ref["virial"][:, b, a]).max()
+                            / np.abs(ref["virial"][:, b, a]).max())
+        print("  fp32 worst component err %.3e" % worst, flush=True)
+        assert worst <= 1e-5
+        assert st["n_ball"] == ref["n_ball"]
+
+
+def test_tile_nonperiodic_y_and_frames_per_launch(gpu):
+    """A box non-periodic along y (one cell row along y, windows of one row)
+    and several frames per launch through compute_frames."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    frames = [workload.make_system(2500, 2, 8500 + f) for f in range(3)]
+    for fr in frames:
+        fr["periodic"] = np.array([1, 0, 0])
+    L = frames[0]["L"]
+    grid = workload.cartesian_grid_2d(L[0], 30)
+    mr = max(1.25 * 1.62, 1.62)
+    e = _engine(2, mr)
+    e.set_box(L, [1, 0, 0])
+    e.set_grid(grid)
+    e.set_tuning(frames_per_launch=3)
+    t = lambda k: torch.tensor(np.stack([f[k] for f in frames]), dtype=torch.float64, device=dev)
+    ov = torch.zeros((3, len(grid), 2, 2), dtype=torch.float64, device=dev)
+    ok = torch.zeros_like(ov)
+    print("tile 2d non-periodic y, 3 frames per launch", flush=True)
+    e.compute_frames(3, 2500, t("pos"), t("diam"), t("mass"), t("vel"), None, fields=("virial", "kinetic"),
+                     d_virial=ov, d_kinetic=ok)
+    torch.cuda.synchronize()
+    nn = 0
+    for f in range(3):
+        ref = oracle.ik_compute(frames[f], grid, fields=("virial", "kinetic"), max_rcut=mr, with_abs=True)
+        nn += ref["n_nonzero"]
+        _check({"virial": ov[f].cpu().numpy(), "kinetic": ok[f].cpu().numpy()}, ref, ("virial", "kinetic"))
+    assert e.stats()["n_contrib"] == nn
+
+
+def test_tile_matches_lane_on_c2_frames(gpu):
+    """Both gathers on 4 frames of the 2D N=65,536 workload (256^2 grid)."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    F, N = 4, 65536
+    frames = [workload.make_system(N, 2, 2000 + f) for f in range(F)]
+    L = frames[0]["L"]
+    grid = workload.cartesian_grid_2d(L[0], 256)
+    mr = max(1.25 * float(frames[0]["diam"].max()), float(frames[0]["diam"].max()))
+    t = lambda k: torch.tensor(np.stack([f[k] for f in frames]), dtype=torch.float64, device=dev)
+    pos, diam, mass, vel = t("pos"), t("diam"), t("mass"), t("vel")
+    res = {}
+    for mode in ("tile", "lane"):
+        e = _engine(2, mr, mode=mode)
+        e.set_box(L, [1, 1, 0])
+        e.set_grid(grid)
+        e.set_tuning(frames_per_launch=F)
+        ov = torch.zeros((F, len(grid), 2, 2), dtype=torch.float64, device=dev)
+        ok = torch.zeros_like(ov)
+        e.compute_frames(F, N, pos, diam, mass, vel, None, fields=("virial", "kinetic"), d_virial=ov, d_kinetic=ok)
+        torch.cuda.synchronize()
+        res[mode] = (ov.cpu().numpy(), ok.cpu().numpy(), e.stats())
+        print("c2 frames %s: gather %.3f ms" % (mode, res[mode][2]["gather_ms"]), flush=True)
+    ref = oracle.ik_compute(frames[0], grid, fields=("virial",), max_rcut=mr, with_abs=True)
+    ev = scaled_error(res["tile"][0][0], ref["virial"], ref["virial_abs"]).max()
+    print("  tile frame 0 max scaled err %.3e" % ev, flush=True)
+    assert ev <= TOL
+    scale = np.maximum(np.abs(res["lane"][0]), 1e-6 * np.abs(res["lane"][0]).max())
+    assert (np.abs(res["tile"][0] - res["lane"][0]) / scale).max() <= 1e-9
+    assert np.abs(res["tile"][1] - res["lane"][1]).max() <= 1e-13 * np.abs(res["lane"][1]).max()
+    assert res["tile"][2]["n_contrib"] == res["lane"][2]["n_contrib"]
+    assert res["tile"][2]["n_ball"] == res["lane"][2]["n_ball"]

@@ -216,3 +216,108 @@ def test_cell_list_equals_brute_force_and_loops_agree(dim):
         np.testing.assert_allclose(b[k], a[k], atol=1e-12 * np.abs(a[k]).max(), rtol=0)
         assert np.array_equal(c[k], a[k])
     assert a["n_triples"] == b["n_triples"] and a["n_nonzero"] == b["n_nonzero"] and a["n_ball"] == b["n_ball"]
+
+
+# ---------------------------------------------------------------------------
+# elastic virial field: ElasticVirialStress::compute
+# (pyglasstools/ListofCommonObservables.h:86-123): per bond
+#   0.5 * bondval * (F1 (x) (u_ij + r_ij) - F (x) r_ij),
+#   u_ij = u_j - u_i, F = f(|r_ij|) r_ij, F1 = f(|u_ij + r_ij|) (u_ij + r_ij)
+# ---------------------------------------------------------------------------
+def numpy_elastic(sysd, grid, rc=RC, order=5, max_rcut=None, eps=0.2, scaled_rcut=1.25):
+    """Independent restatement of the elastic field on tiny systems (same
+    ingredients as numpy_field: own minimum image, numpy Gauss-Legendre nodes,
+    the Octic as (1 - t^2)^2, the force from a sympy derivative)."""
+    pos = np.asarray(sysd["pos"], float)
+    disp = np.asarray(sysd["disp"], float)
+    L = np.asarray(sysd["L"], float)
+    per = np.asarray(sysd["periodic"], bool)
+    diam = np.asarray(sysd["diam"], float)
+    D = sysd["dim"]
+    if max_rcut is None:
+        max_rcut = max(scaled_rcut * diam.max(), diam.max())
+
+    def mi(d):
+        d = d.copy()
+        d[..., per] -= L[per] * np.round(d[..., per] / L[per])
+        return d
+
+    def fdr(r, sig):
+        return FDR(r, sig) if r < scaled_rcut * sig else 0.0
+
+    z, w = np.polynomial.legendre.leggauss(order)
+    s, ws = 0.5 * (z + 1), 0.5 * w
+    norm = 15 / (8 * math.pi * rc * rc)
+    T = np.zeros((len(grid), D, D))
+    for g, x in enumerate(np.asarray(grid, float)):
+        dr = mi(x - pos)
+        for i in np.nonzero((dr ** 2).sum(1) < rc * rc)[0]:
+            ri = x - dr[i]
+            rij = mi(pos - pos[i])
+            r2 = (rij ** 2).sum(1)
+            sel = (np.arange(len(pos)) != i) & (r2 < max_rcut ** 2)
+            for j in np.nonzero(sel)[0]:
+                sig = 0.5 * (diam[i] + diam[j]) * (1 - eps * abs(diam[i] - diam[j]))
+                r = rij[j]
+                u = disp[j] - disp[i]
+                sv = u + r
+                f = fdr(math.sqrt(r @ r), sig)
+                f1 = fdr(math.sqrt(sv @ sv), sig)
+                pts = ri[None, :] + s[:, None] * r[None, :]
+                t = ((x - pts) ** 2).sum(1) / (rc * rc)
+                bond = float(ws @ np.where(t < 1, norm * (1 - t * t) ** 2, 0.0))
+                T[g] += 0.5 * bond * (f1 * np.outer(sv[:D], sv[:D]) - f * np.outer(r[:D], r[:D])).T
+    return T
+
+
+def _with_disp(sysd, scale, seed):
+    rng = np.random.default_rng(seed)
+    u = rng.normal(0.0, scale, sysd["pos"].shape)
+    if sysd["dim"] == 2:
+        u[:, 2] = 0.0
+    return dict(sysd, disp=u)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_elastic_independent_numpy_restatement(dim):
+    sysd = small_system(50 if dim == 2 else 130, 11, dim=dim)
+    sysd = _with_disp(sysd, 0.05, 12)
+    grid = workload.cartesian_grid_2d(sysd["L"][0], 5) if dim == 2 else workload.cartesian_grid_3d(sysd["L"][0], 3)
+    ref = numpy_elastic(sysd, grid)
+    got = oracle.ik_compute(sysd, grid, fields=("elastic",))["elastic"]
+    scale = np.abs(ref).max()
+    assert scale > 0.1
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-11 * scale)
+
+
+def test_elastic_zero_displacement_is_exactly_zero():
+    """u = 0 gives F1 = F bond by bond, so every term cancels exactly while
+    the virial field of the same system is not zero."""
+    sysd = dict(small_system(60, 13), disp=np.zeros((60, 3)))
+    grid = workload.cartesian_grid_2d(sysd["L"][0], 6)
+    out = oracle.ik_compute(sysd, grid, fields=("virial", "elastic"))
+    assert np.abs(out["virial"]).max() > 1.0
+    assert np.all(out["elastic"] == 0.0)
+
+
+def test_elastic_two_particles_closed_form():
+    """Two particles inside the ball of the grid point, displaced: the two
+    bonds (0 -> 1 and 1 -> 0) each add half of w (f(|s|) s s - f(|r|) r r)
+    with s = r_01 + u_1 - u_0, w the exact inside integral.  The values are
+    chosen so that a sign error in u_ij (u_i - u_j) or a dropped factor 1/2
+    changes the result by far more than rounding."""
+    sysd = _pair_system(-0.5, 0.5)
+    u = np.array([[0.03, -0.02, 0.0], [-0.04, 0.05, 0.0]])
+    sysd["disp"] = u
+    T = oracle.ik_compute(sysd, np.array([[0.0, 0.0, 0.0]]), fields=("elastic",))["elastic"][0]
+    r = np.array([1.0, 0.0, 0.0])
+    s = r + u[1] - u[0]
+    f = FDR(1.0, 1.0)
+    f1 = FDR(math.sqrt(s @ s), 1.0)
+    w = _exact_inside([0, 0, 0], [-0.5, 0, 0], r)
+    want = w * (f1 * np.outer(s[:2], s[:2]) - f * np.outer(r[:2], r[:2]))
+    np.testing.assert_allclose(T, want.T, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    # the sign of u_ij matters: the result with u_i - u_j differs clearly
+    s_wrong = r - (u[1] - u[0])
+    wrong = w * (FDR(math.sqrt(s_wrong @ s_wrong), 1.0) * np.outer(s_wrong[:2], s_wrong[:2]) - f * np.outer(r[:2], r[:2]))
+    assert np.abs(wrong - want).max() > 1e-3 * np.abs(want).max()
