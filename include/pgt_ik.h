@@ -155,6 +155,25 @@ int pgt_ik_compute_frames(pgt_ik_engine *e, int nframes, int64_t n, const double
                           const double *d_disp, unsigned fields, double *d_out_virial,
                           double *d_out_kinetic, double *d_out_elastic);
 
+/* pgt_ik_compute_frames that allocates nothing.  No counterpart in the
+ * reference: it serves callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and statistics as pgt_ik_compute_frames, but every
+ * work array (cell list, bond list, statistics) lives in a static device
+ * array of 64 MiB per GPU, held until the process ends; the call makes no
+ * hipMalloc, hipFree or other allocating call.  Frames are run in launches of
+ * at most pgt_ik_set_tuning's frames-per-launch that fit the array; a job
+ * whose single frame needs more work memory than that returns PGT_ERR_RANGE
+ * (2D FP64 virial + kinetic: about 450 bytes per particle, so up to ~140,000
+ * particles per frame).  The engine itself (pgt_ik_create, pgt_ik_set_grid)
+ * allocates at set-up as before.  All engines on one GPU share its array and
+ * it keeps the last call's contents: calls that overlap on one GPU race, so
+ * serialise them.  Synchronises the engine's stream. */
+int pgt_ik_compute_frames_noalloc(pgt_ik_engine *e, int nframes, int64_t n, const double *d_pos,
+                                  const double *d_diam, const double *d_mass, const double *d_vel,
+                                  const double *d_disp, unsigned fields, double *d_out_virial,
+                                  double *d_out_kinetic, double *d_out_elastic);
+
 /* Enqueue a job like pgt_ik_compute_frames and return at once: no host
  * synchronisation, no device allocation, no memset, no events, so a caller
  * can capture the launches into a HIP graph (hipStreamBeginCapture on the

@@ -27,6 +27,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -292,10 +293,46 @@ struct pgt_ik_engine {
     // change the bond capacity the launches were built with) until
     // pgt_ik_release_enqueued
     bool held = false;
+    // the no-allocation entry point: this device's copy of the static scratch
+    // array (address from hipGetSymbolAddress at create) and the bond
+    // capacity that entry point uses
+    unsigned char *scratch = nullptr;
+    int scratch_cap = 0;
     pgt_ik_stats stats{};
 };
 
+// Scratch memory of pgt_ik_compute_frames_noalloc: every work array of its
+// batches (cell list, bond list, statistics) is carved out of this array, so
+// that entry point never allocates.  Each GPU the library runs on holds its
+// own copy for the life of the process.  In host code `scratch` names a
+// host-side stand-in, not device memory: the device address comes from
+// hipGetSymbolAddress (scratch_on_device), made once per device with that
+// device current, and only that address is passed to kernels.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+std::mutex g_scratch_mu;
+std::vector<unsigned char *> g_scratch_addr;  // per device id
+
+// Device address of the scratch array on device `dev`, which must be the
+// current device.
+int scratch_on_device(int dev, unsigned char **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0) return fail(PGT_ERR_ARG, "bad device id");
+    if ((int)g_scratch_addr.size() <= dev) g_scratch_addr.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_addr[(size_t)dev]) {
+        void *ptr = nullptr;
+        const hipError_t er = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (er != hipSuccess || !ptr)
+            return fail(PGT_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(er));
+        g_scratch_addr[(size_t)dev] = static_cast<unsigned char *>(ptr);
+    }
+    *out = g_scratch_addr[(size_t)dev];
+    return 0;
+}
 
 int check_engine(pgt_ik_engine *e)
 {
@@ -965,6 +1002,10 @@ int pgt_ik_create(const pgt_ik_params *params, pgt_ik_engine **out)
     pgt_ik_engine *e = new pgt_ik_engine();
     e->prm = *params;
     (void)hipGetDevice(&e->device);
+    if ((rc = scratch_on_device(e->device, &e->scratch))) {
+        delete e;
+        return rc;
+    }
     hipError_t he = hipStreamCreateWithFlags(&e->own_stream, hipStreamNonBlocking);
     if (he != hipSuccess) {
         delete e;
@@ -1327,6 +1368,99 @@ int pgt_ik_compute_frames(pgt_ik_engine *e, int nframes, int64_t n, const double
     // flat: positions' z components are not read for such frames.
     fs.z_flat = e->grid_z_flat;
     return run_job(e, fs, fields);
+}
+
+int pgt_ik_compute_frames_noalloc(pgt_ik_engine *e, int nframes, int64_t n, const double *d_pos,
+                                  const double *d_diam, const double *d_mass, const double *d_vel,
+                                  const double *d_disp, unsigned fields, double *d_out_virial,
+                                  double *d_out_kinetic, double *d_out_elastic)
+{
+    if (int rc = check_engine(e)) return rc;
+    if (!e->box_set || !e->have_grid)
+        return fail(PGT_ERR_STATE, "set_box and set_grid must precede compute_frames_noalloc");
+    if (nframes < 0 || n < 0 || n > (int64_t)1 << 30) return fail(PGT_ERR_ARG, "bad frame or particle count");
+    if (fields == 0 || (fields & ~7u)) return fail(PGT_ERR_ARG, "bad field mask");
+    if (((fields & PGT_FIELD_VIRIAL) && !d_out_virial) || ((fields & PGT_FIELD_KINETIC) && !d_out_kinetic) ||
+        ((fields & PGT_FIELD_ELASTIC) && !d_out_elastic))
+        return fail(PGT_ERR_ARG, "null output for a selected field");
+    if (n > 0 && (!d_pos || !d_diam)) return fail(PGT_ERR_ARG, "positions and diameters are required");
+    if (!e->scratch) return fail(PGT_ERR_STATE, "no scratch array on the engine's device");
+    e->stats = pgt_ik_stats{};
+    if (nframes == 0 || e->G == 0) return PGT_OK;
+    const int OD = e->prm.dim;
+    const size_t fsz = (size_t)e->G * OD * OD;
+    hipStream_t s = e->stream;
+    if (n == 0) {  // fill kernels, no memset
+        const unsigned bits[3] = {PGT_FIELD_VIRIAL, PGT_FIELD_KINETIC, PGT_FIELD_ELASTIC};
+        double *outs[3] = {d_out_virial, d_out_kinetic, d_out_elastic};
+        for (int k = 0; k < 3; ++k)
+            if (fields & bits[k])
+                if (int rc = fill_async(outs[k], 0.0, fsz * nframes, s)) return rc;
+        HIPCHK(hipStreamSynchronize(s));
+        return PGT_OK;
+    }
+    FrameSet fs{};
+    fs.nframes = nframes;
+    fs.n = n;
+    fs.pos = d_pos;
+    fs.diam = d_diam;
+    fs.mass = d_mass;
+    fs.vel = d_vel;
+    fs.disp = d_disp;
+    fs.vir = (fields & PGT_FIELD_VIRIAL) ? d_out_virial : nullptr;
+    fs.kin = (fields & PGT_FIELD_KINETIC) ? d_out_kinetic : nullptr;
+    fs.ela = (fields & PGT_FIELD_ELASTIC) ? d_out_elastic : nullptr;
+    fs.z_flat = e->grid_z_flat;
+    JobShape j = job_shape(e, fields, fs.z_flat);
+    int cap = e->scratch_cap;
+    if (j.pairs && cap < 1) cap = j.D == 3 ? 32 : 16;
+    if (!j.pairs) cap = 1;
+    int mx = 0;
+    unsigned long long h[2] = {0, 0};
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        // the most frames per launch whose work arrays fit the scratch array
+        int nf = std::min(std::max(1, e->frames_per_launch), nframes);
+        while (nf > 0 && work_carve(j, nf, n, e->G, cap, nullptr, nullptr) > kScratchBytes) nf = nf / 2;
+        if (nf < 1)
+            return fail(PGT_ERR_RANGE, "one frame of this job needs " +
+                                           std::to_string(work_carve(j, 1, n, e->G, cap, nullptr, nullptr)) +
+                                           " bytes of work memory; the no-allocation entry point has " +
+                                           std::to_string(kScratchBytes) + " (use pgt_ik_compute_frames)");
+        Work w{};
+        work_carve(j, nf, n, e->G, cap, e->scratch, &w);
+        // counters start at zero on every call (the array keeps the last call's contents)
+        int rc;
+        if ((rc = fill_async(w.maxcount, 0, 2, s)) || (rc = fill_async(w.stats, 0ull, 2, s))) return rc;
+        for (int f0 = 0; f0 < nframes; f0 += nf)
+            if ((rc = run_batch(e, j, w, cap, fs, fields, f0, std::min(nf, nframes - f0), nullptr, nullptr, nullptr)))
+                return rc;
+        int flags[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(flags, w.maxcount, sizeof(flags), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h, w.stats, sizeof(h), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        mx = flags[0];
+        if (flags[1] & 1)
+            return fail(PGT_ERR_RANGE, "an adaptive bond integral needed more than " + std::to_string(kQagMax) +
+                                           " intervals (the GPU rule's workspace)");
+        const bool grow = j.pairs && mx > cap;
+        if (grow && mx > 65535) return fail(PGT_ERR_RANGE, "a particle has more than 65535 interacting neighbours");
+        if (flags[1] & 6) {
+            if (!j.tile) return fail(PGT_ERR_STATE, "tile overflow reported by a lane gather");
+            j = job_shape(e, fields, fs.z_flat, false);
+        } else if (!grow) {
+            break;
+        }
+        if (grow) cap = ((mx + 7) / 8) * 8;
+        if (attempt == 2) return fail(PGT_ERR_HIP, "bond list capacity did not settle");
+    }
+    if (j.pairs) e->scratch_cap = cap;
+    e->stats.n_particles = n;
+    e->stats.n_grid = e->G;
+    e->stats.n_ball = (int64_t)h[0];
+    e->stats.n_contrib = (int64_t)h[1];
+    e->stats.bond_capacity = cap;
+    e->stats.max_bonds = mx;
+    return PGT_OK;
 }
 
 int pgt_eval_pair_force(int potential_kind, const double *pot_params, int n_pot_params, double scaled_rcut, int64_t n,

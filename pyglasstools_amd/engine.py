@@ -22,7 +22,8 @@ MLP_IN_PAD = 8  # floats per network input row (kMlpInPad)
 # every symbol include/pgt_ik.h declares
 EXPORTED = [
     "pgt_last_error", "pgt_device_count", "pgt_ik_create", "pgt_ik_destroy", "pgt_ik_set_box",
-    "pgt_ik_set_particles", "pgt_ik_set_grid", "pgt_ik_compute", "pgt_ik_compute_frames", "pgt_ik_enqueue_frames",
+    "pgt_ik_set_particles", "pgt_ik_set_grid", "pgt_ik_compute", "pgt_ik_compute_frames",
+    "pgt_ik_compute_frames_noalloc", "pgt_ik_enqueue_frames",
     "pgt_ik_finish", "pgt_ik_release_enqueued", "pgt_ik_set_stream",
     "pgt_ik_set_tuning", "pgt_ik_set_gather_mode", "pgt_ik_get_stats", "pgt_eval_pair_force", "pgt_eval_cgfunc",
     "pgt_mlp_nparam", "pgt_mlp_work_floats", "pgt_mlp_forward", "pgt_mlp_grad", "pgt_mlp_adam", "pgt_mlp_infer",
@@ -89,6 +90,7 @@ def lib():
         L.pgt_ik_compute.argtypes = [vp, ctypes.c_uint, vp, vp, vp]
         L.pgt_ik_compute_frames.argtypes = [vp, i, i64, vp, vp, vp, vp, vp, ctypes.c_uint, vp, vp, vp]
         L.pgt_ik_enqueue_frames.argtypes = [vp, i, i64, vp, vp, vp, vp, vp, ctypes.c_uint, vp, vp, vp]
+        L.pgt_ik_compute_frames_noalloc.argtypes = [vp, i, i64, vp, vp, vp, vp, vp, ctypes.c_uint, vp, vp, vp]
         L.pgt_ik_finish.argtypes = [vp]
         L.pgt_ik_release_enqueued.argtypes = [vp]
         L.pgt_ik_set_stream.argtypes = [vp, vp]
@@ -316,6 +318,15 @@ class Engine(object):
         nframes, n, mask, ptrs, outs = self._frame_args(nframes, n, d_pos, d_diam, d_mass, d_vel, d_disp, fields,
                                                         d_virial, d_kinetic, d_elastic)
         _check(lib().pgt_ik_compute_frames(self.h, nframes, n, *ptrs, mask, *outs), "pgt_ik_compute_frames")
+
+    def compute_frames_noalloc(self, nframes, n, d_pos, d_diam, d_mass=None, d_vel=None, d_disp=None,
+                               fields=("virial",), d_virial=None, d_kinetic=None, d_elastic=None):
+        """compute_frames with every work array in the library's static
+        device scratch array: the call allocates nothing."""
+        nframes, n, mask, ptrs, outs = self._frame_args(nframes, n, d_pos, d_diam, d_mass, d_vel, d_disp, fields,
+                                                        d_virial, d_kinetic, d_elastic)
+        _check(lib().pgt_ik_compute_frames_noalloc(self.h, nframes, n, *ptrs, mask, *outs),
+               "pgt_ik_compute_frames_noalloc")
 
     def set_stream(self, stream_handle):
         _check(lib().pgt_ik_set_stream(self.h, ctypes.c_void_p(stream_handle) if stream_handle else None),

@@ -30,7 +30,7 @@ def test_dev_accepts_none_and_rejects_host_memory():
         engine._dev(12345678, torch.float64, 10, "x")
 
 
-@pytest.mark.parametrize("method", ["compute_frames", "enqueue_frames"])
+@pytest.mark.parametrize("method", ["compute_frames", "enqueue_frames", "compute_frames_noalloc"])
 def test_frame_calls_reject_bad_device_arguments(method):
     e = fake_engine()
     nf, n = 2, 5
