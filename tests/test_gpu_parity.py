@@ -715,9 +715,10 @@ def test_graph_captured_batches_match_eager(gpu, forked):
 
 def test_enqueued_buffers_are_held_until_released(gpu):
     """After enqueue_frames the engine refuses calls that would free, grow or
-    replace the buffers a captured graph replays (set_grid, a larger
-    compute_frames), with PGT_ERR_STATE, until release_enqueued; a job of the
-    same shape still runs.  Host-side checks only: nothing faulting is run."""
+    replace the buffers a captured graph replays (set_grid with another grid,
+    a larger compute_frames), with PGT_ERR_STATE, until release_enqueued; a
+    job of the same shape, and the same grid, are accepted.  Host-side checks
+    only: nothing faulting is run."""
     import torch
 
     dev = torch.device("cuda:0")
@@ -735,8 +736,9 @@ def test_enqueued_buffers_are_held_until_released(gpu):
     e.compute_frames(nf, N, pos, diam, d_virial=ov)
     e.enqueue_frames(nf, N, pos, diam, d_virial=ov)
     e.finish()
+    e.set_grid(grid)  # the grid already set: nothing changes, accepted
     with pytest.raises(engine.PGTError, match="PGT_ERR_STATE"):
-        e.set_grid(grid)
+        e.set_grid(grid + np.array([0.01, 0.0, 0.0]))
     # more frames per launch needs larger buffers: refused
     e.set_tuning(frames_per_launch=nf + 2)
     pos4, diam4 = t("pos", nf + 2), t("diam", nf + 2)
