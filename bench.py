@@ -230,12 +230,16 @@ def precision_form(args):
     from pyglasstools_amd.autotune import Autotuner
 
     if args.block and args.pl:
+        mode = "tile" if args.gather == "auto" else args.gather
+        eng.set_gather_mode(mode)
         eng.set_tuning(args.block, F, args.pl, args.reach)
-        tuned = {"threads_per_block": args.block, "frames_per_launch": F, "points_per_lane": args.pl,
-                 "cell_reach": args.reach}
+        tuned = {"gather": mode, "threads_per_block": args.block, "frames_per_launch": F,
+                 "points_per_lane": args.pl, "cell_reach": args.reach}
     else:
-        log("autotuning threads per workgroup x points per lane")
-        tuned, _ = Autotuner().tune(eng, step, ("c3", N, G, F), F, fpl=(F,), sync=torch.cuda.synchronize)
+        log("autotuning gather kind x threads per workgroup x points per lane")
+        gathers = ("lane", "tile") if args.gather == "auto" else (args.gather,)
+        tuned, _ = Autotuner().tune(eng, step, ("c3", N, G, F, args.gather), F, fpl=(F,),
+                                    sync=torch.cuda.synchronize, gathers=gathers)
         log("tuned: %s" % tuned)
     for _ in range(args.warmup):
         step()
@@ -283,7 +287,8 @@ def precision_form(args):
                        "frames_per_step": F, "n_particles": N, "grid": "64^3", "parallelism": "single GPU"},
             "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                         "kernel": "k_gather (field gather, FP32 form)",
+                         "kernel": ("k_gather_tile" if tuned["gather"] == "tile" else "k_gather") +
+                                   " (field gather, FP32 form)",
                          "compute": "FP32 vector ALU (published MI355X FP32 peak 157.3 TFLOP/s, which counts packed "
                                     "two-wide FP32; this build issues unpacked FP32, whose rate equals the FP64 "
                                     "rate of 78.6 TFLOP/s)",
@@ -579,8 +584,10 @@ def main():
     ap.add_argument("--fpl", type=int, default=0, help="frames per launch (0: autotune)")
     ap.add_argument("--pl", type=int, default=0, help="grid points per gather lane (0: autotune)")
     ap.add_argument("--reach", type=int, default=1, help="cell reach with --block/--fpl/--pl fixed (1 or 2)")
-    ap.add_argument("--gather", default="tile", choices=["tile", "lane"],
-                    help="field gather for the Octic 5-node rule: LDS-staged tiles or one lane per point")
+    ap.add_argument("--gather", default="auto", choices=["auto", "tile", "lane"],
+                    help="field gather for the Octic 5-node rule: item-balanced 64-point tiles, one lane per point "
+                         "(or group of points), or the faster of the two as the autotuner finds (auto; tile with "
+                         "--block/--pl fixed)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (reference dispatches for HBM byte counters) and exit")
     args = ap.parse_args()
@@ -640,7 +647,6 @@ def main():
     eng = engine.Engine(dim=2, max_rcut=max_rcut, precision=args.precision)
     eng.set_box(L, [1, 1, 0])
     eng.set_grid(grid)
-    eng.set_gather_mode(args.gather)
 
     def step():
         eng.compute_frames(F, N, pos, diam, mass, vel, None, fields=("virial", "kinetic"), d_virial=out_v,
@@ -648,13 +654,18 @@ def main():
 
     tuned, trials = None, []
     if args.block and args.fpl and args.pl:
+        mode = "tile" if args.gather == "auto" else args.gather
+        eng.set_gather_mode(mode)
         eng.set_tuning(args.block, args.fpl, args.pl, args.reach)
-        tuned = {"threads_per_block": args.block, "frames_per_launch": args.fpl, "points_per_lane": args.pl,
-                 "cell_reach": args.reach}
+        tuned = {"gather": mode, "threads_per_block": args.block, "frames_per_launch": args.fpl,
+                 "points_per_lane": args.pl, "cell_reach": args.reach}
     else:
-        log("autotuning cell reach x points per lane x threads per workgroup x frames per launch")
-        tuned, trials = Autotuner().tune(eng, step, ("c2", N, G, F, args.precision), F, sync=torch.cuda.synchronize)
+        log("autotuning gather kind x cell reach x points per lane x threads per workgroup x frames per launch")
+        gathers = ("lane", "tile") if args.gather == "auto" else (args.gather,)
+        tuned, trials = Autotuner().tune(eng, step, ("c2", N, G, F, args.precision, args.gather), F,
+                                         sync=torch.cuda.synchronize, gathers=gathers)
         log("tuned: %s" % tuned)
+    mode = tuned["gather"]
 
     for _ in range(args.warmup):
         step()
@@ -677,10 +688,10 @@ def main():
     flops = FLOP_U_2D * n_contrib + FLOP_KIN_2D * n_ball
     avg_launch_s = gather_ms / 1e3 / launches
     achieved = flops / launches / avg_launch_s / 1e12
-    traffic, traffic_fpl = traffic_from_profile(args.gather)
+    traffic, traffic_fpl = traffic_from_profile(mode)
     if traffic is not None and traffic_fpl != fpl:
         traffic = traffic * fpl / traffic_fpl  # per-launch bytes scale with frames per launch
-    kname = "k_gather_tile (LDS-staged field gather)" if args.gather == "tile" else "k_gather (lane field gather)"
+    kname = "k_gather_tile (item-balanced tile field gather)" if mode == "tile" else "k_gather (lane field gather)"
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                 "kernel": kname, "compute": "FP64 vector ALU (published MI355X FP64 peak 78.6 TFLOP/s)",

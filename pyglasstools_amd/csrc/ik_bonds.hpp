@@ -128,9 +128,9 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
                 const int q = qq[u];
                 const double dj = djs[u];
                 double b[3];
-                b[0] = minimg(cqs[u][0] - xi, g.L[0], g.invL[0], g.periodic[0]);
-                b[1] = minimg(cqs[u][1] - yi, g.L[1], g.invL[1], g.periodic[1]);
-                b[2] = (D == 3) ? minimg(cqs[u][2] - zi, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                b[0] = minimg(cqs[u][0] - xi, g.L[0], g.pinvL[0]);
+                b[1] = minimg(cqs[u][1] - yi, g.L[1], g.pinvL[1]);
+                b[2] = (D == 3) ? minimg(cqs[u][2] - zi, g.L[2], g.pinvL[2]) : 0.0;
                 const double rsq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
                 double f0, f1 = 0.0;
                 double ur[3] = {0, 0, 0};
@@ -207,9 +207,9 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
                 }
                 // exact test: FP64 minimum image and |r_ij|^2 against max_rcut^2
                 auto within = [&](const double *cq) {
-                    const double b0 = minimg(cq[0] - xi, g.L[0], g.invL[0], g.periodic[0]);
-                    const double b1 = minimg(cq[1] - yi, g.L[1], g.invL[1], g.periodic[1]);
-                    const double b2 = (D == 3) ? minimg(cq[2] - zi, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                    const double b0 = minimg(cq[0] - xi, g.L[0], g.pinvL[0]);
+                    const double b1 = minimg(cq[1] - yi, g.L[1], g.pinvL[1]);
+                    const double b2 = (D == 3) ? minimg(cq[2] - zi, g.L[2], g.pinvL[2]) : 0.0;
                     return b0 * b0 + b1 * b1 + b2 * b2 < g.mr2;
                 };
                 if constexpr (D == 3) {
@@ -228,9 +228,9 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
                         const int q = q0 + u;
                         if (q >= re[s]) break;
                         float d0 = c32[u][0] - xi32, d1 = c32[u][1] - yi32, d2 = c32[u][2] - zi32;
-                        if (g.periodic[0]) d0 = fmaf(-g.L32[0], rintf(d0 * g.invL32[0]), d0);
-                        if (g.periodic[1]) d1 = fmaf(-g.L32[1], rintf(d1 * g.invL32[1]), d1);
-                        if (g.periodic[2]) d2 = fmaf(-g.L32[2], rintf(d2 * g.invL32[2]), d2);
+                        d0 = fmaf(-g.L32[0], rintf(d0 * g.pinvL32[0]), d0);
+                        d1 = fmaf(-g.L32[1], rintf(d1 * g.pinvL32[1]), d1);
+                        d2 = fmaf(-g.L32[2], rintf(d2 * g.pinvL32[2]), d2);
                         const float a2f = d0 * d0 + d1 * d1 + d2 * d2;
                         bool in = a2f < g.bin32;
                         if (!in && !(a2f > g.bout32)) {

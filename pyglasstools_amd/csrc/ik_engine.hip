@@ -287,7 +287,7 @@ struct pgt_ik_engine {
     int frames_per_launch = 8;
     int points_per_lane = 2;  // grid points per gather lane (Octic 5-node rule; 1 elsewhere)
     int cell_reach = 1;       // cells searched on each side (cell edge >= search radius / reach)
-    int gather_mode = 0;      // 1: LDS-staged tile gather where the form has one, 0: one lane per point
+    int gather_mode = 0;      // 1: item-balanced tile gather where the form has one, 0: one lane per point
     // set by pgt_ik_enqueue_frames: launches that a captured graph may replay
     // hold the current device buffers, so nothing may free or move them (or
     // change the bond capacity the launches were built with) until
@@ -415,6 +415,7 @@ Geom make_geom(const pgt_ik_engine *e, int D)
         g.invL[a] = 1.0 / e->L[a];
         g.lo[a] = e->lo[a];
         g.periodic[a] = e->periodic[a];
+        g.pinvL[a] = e->periodic[a] ? g.invL[a] : 0.0;
         int nc = (int)std::floor(e->L[a] * R / rmax);
         if (!e->periodic[a] || nc < 2 * R + 1 || (a == 2 && D == 2)) nc = 1;
         g.nc[a] = nc;
@@ -437,6 +438,7 @@ Geom make_geom(const pgt_ik_engine *e, int D)
     for (int a = 0; a < 3; ++a) {
         g.L32[a] = (float)g.L[a];
         g.invL32[a] = (float)g.invL[a];
+        g.pinvL32[a] = g.periodic[a] ? g.invL32[a] : 0.0f;
     }
     bool ok = std::isfinite(e->grid_absmax);
     double M = e->grid_absmax, Lm = 0.0, Lmin = INFINITY;
@@ -507,13 +509,13 @@ struct JobShape {
     int npass, nbits, radix;
     size_t ebytes;
     int block, pl;
-    bool tile;       // the LDS-staged tile gather (kTilePoints points per workgroup of kTileThreads lanes)
+    bool tile;       // the item-balanced tile gather (kTilePoints points per workgroup of kTileThreads lanes)
     int64_t gwaves;  // gather wavefronts per frame
 };
 
-// allow_tile false: the lane gather even where the tile gather is tuned in
-// (the fallback when a tile window overflows the LDS arrays)
-JobShape job_shape(const pgt_ik_engine *e, unsigned fields, bool z_flat, bool allow_tile = true)
+// n: particles per frame.  allow_tile false: the lane gather even where the
+// tile gather is selected (the fallback when a tile reports an overflow)
+JobShape job_shape(const pgt_ik_engine *e, unsigned fields, bool z_flat, int64_t n, bool allow_tile = true)
 {
     JobShape j{};
     j.OD = e->prm.dim;
@@ -539,7 +541,7 @@ JobShape job_shape(const pgt_ik_engine *e, unsigned fields, bool z_flat, bool al
     else j.ebytes = j.fp32 ? bond_entry_bytes<3, 3, float>(fields) : bond_entry_bytes<3, 3, double>(fields);
     j.block = j.adaptive ? 128 : e->block;  // k_gather_adaptive runs 128-thread workgroups
     j.pl = j.adaptive ? 1 : gather_pl(e->points_per_lane, e->prm.cg_kind, (int)e->gl_s.size(), j.D, j.OD);
-    j.tile = allow_tile && !j.adaptive && e->gather_mode == 1 &&
+    j.tile = allow_tile && !j.adaptive && e->gather_mode == 1 && n <= kTileMaxParticles &&
              tile_form(e->prm.cg_kind, (int)e->gl_s.size(), j.D, j.OD);
     if (j.tile) {
         j.block = kTileThreads;
@@ -808,7 +810,7 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
 {
     const int fpl = std::max(1, e->frames_per_launch);
     const int nb = (fs.nframes + fpl - 1) / fpl;
-    JobShape j = job_shape(e, fields, fs.z_flat);
+    JobShape j = job_shape(e, fields, fs.z_flat, fs.n);
     int rc;
     if ((rc = ensure_events(e, (size_t)nb * 3))) return rc;
     if ((rc = e->d_maxcount.ensure(2)) || (rc = e->d_stats.ensure(2))) return rc;
@@ -838,7 +840,7 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
             // a tile window held more particles or entries than the LDS
             // arrays: run the job again with the lane gather
             if (!j.tile) return fail(PGT_ERR_STATE, "tile overflow reported by a lane gather");
-            j = job_shape(e, fields, fs.z_flat, false);
+            j = job_shape(e, fields, fs.z_flat, fs.n, false);
         } else if (!grow) {
             break;
         }
@@ -876,7 +878,7 @@ int enqueue_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
     const int nb = (fs.nframes + fpl - 1) / fpl;
     if (e->cap < 1 || !e->d_maxcount.p || e->d_maxcount.n < 2 || !e->d_stats.p)
         return fail(PGT_ERR_STATE, "run pgt_ik_compute_frames of the same shape before enqueueing");
-    const JobShape j = job_shape(e, fields, fs.z_flat);
+    const JobShape j = job_shape(e, fields, fs.z_flat, fs.n);
     Work w{};
     if (int rc = work_from_engine(e, j, std::min(fpl, fs.nframes), fs.n, true, w)) return rc;
     for (int b = 0; b < nb; ++b) {
@@ -1411,7 +1413,7 @@ int pgt_ik_compute_frames_noalloc(pgt_ik_engine *e, int nframes, int64_t n, cons
     fs.kin = (fields & PGT_FIELD_KINETIC) ? d_out_kinetic : nullptr;
     fs.ela = (fields & PGT_FIELD_ELASTIC) ? d_out_elastic : nullptr;
     fs.z_flat = e->grid_z_flat;
-    JobShape j = job_shape(e, fields, fs.z_flat);
+    JobShape j = job_shape(e, fields, fs.z_flat, fs.n);
     int cap = e->scratch_cap;
     if (j.pairs && cap < 1) cap = j.D == 3 ? 32 : 16;
     if (!j.pairs) cap = 1;
@@ -1446,7 +1448,7 @@ int pgt_ik_compute_frames_noalloc(pgt_ik_engine *e, int nframes, int64_t n, cons
         if (grow && mx > 65535) return fail(PGT_ERR_RANGE, "a particle has more than 65535 interacting neighbours");
         if (flags[1] & 6) {
             if (!j.tile) return fail(PGT_ERR_STATE, "tile overflow reported by a lane gather");
-            j = job_shape(e, fields, fs.z_flat, false);
+            j = job_shape(e, fields, fs.z_flat, fs.n, false);
         } else if (!grow) {
             break;
         }

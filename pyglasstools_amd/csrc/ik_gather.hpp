@@ -292,9 +292,9 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                 ER ta[PL][3], Aj[PL];
 #pragma unroll
                 for (int j = 0; j < PL; ++j) {
-                    const double a0 = minimg(pos[0] - xg[j][0], g.L[0], g.invL[0], g.periodic[0]);
-                    const double a1 = minimg(pos[1] - xg[j][1], g.L[1], g.invL[1], g.periodic[1]);
-                    const double a2 = (D == 3) ? minimg(pos[2] - xg[j][2], g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                    const double a0 = minimg(pos[0] - xg[j][0], g.L[0], g.pinvL[0]);
+                    const double a1 = minimg(pos[1] - xg[j][1], g.L[1], g.pinvL[1]);
+                    const double a2 = (D == 3) ? minimg(pos[2] - xg[j][2], g.L[2], g.pinvL[2]) : 0.0;
                     const double A = (a0 * a0 + a1 * a1 + a2 * a2) * g.inv_rc2;
                     if (KIN && (PL == 1 || ((m >> j) & 1))) {
                         const double w = shape<CG, double>(A);
@@ -382,10 +382,10 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                             int mm = 0;
 #pragma unroll
                             for (int j = 0; j < PL; ++j) {
-                                const double a0 = minimg(qq[0] - xg[j][0], g.L[0], g.invL[0], g.periodic[0]);
-                                const double a1 = minimg(qq[1] - xg[j][1], g.L[1], g.invL[1], g.periodic[1]);
+                                const double a0 = minimg(qq[0] - xg[j][0], g.L[0], g.pinvL[0]);
+                                const double a1 = minimg(qq[1] - xg[j][1], g.L[1], g.pinvL[1]);
                                 const double a2 =
-                                    (D == 3) ? minimg(qq[2] - xg[j][2], g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                                    (D == 3) ? minimg(qq[2] - xg[j][2], g.L[2], g.pinvL[2]) : 0.0;
                                 const double asq = a0 * a0 + a1 * a1 + a2 * a2;
                                 if (pv[j] && asq < g.rc2) mm |= 1 << j;
                             }
@@ -420,11 +420,11 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
 #pragma unroll
                                 for (int j = 0; j < PL; ++j) {
                                     float d0 = r4[u].x - xg32[j][0], d1 = r4[u].y - xg32[j][1], d2 = 0.0f;
-                                    if (g.periodic[0]) d0 = fmaf(-g.L32[0], rintf(d0 * g.invL32[0]), d0);
-                                    if (g.periodic[1]) d1 = fmaf(-g.L32[1], rintf(d1 * g.invL32[1]), d1);
+                                    d0 = fmaf(-g.L32[0], rintf(d0 * g.pinvL32[0]), d0);
+                                    d1 = fmaf(-g.L32[1], rintf(d1 * g.pinvL32[1]), d1);
                                     if (D == 3) {
                                         d2 = r4[u].z - xg32[j][2];
-                                        if (g.periodic[2]) d2 = fmaf(-g.L32[2], rintf(d2 * g.invL32[2]), d2);
+                                        d2 = fmaf(-g.L32[2], rintf(d2 * g.pinvL32[2]), d2);
                                     }
                                     const float a2f = d0 * d0 + d1 * d1 + d2 * d2;
                                     if (pv[j]) {

@@ -64,9 +64,9 @@ __global__ __launch_bounds__(128) void k_gather_adaptive(Geom g, int n, int G, c
             for (int s = 0; s < nr; ++s)
                 for (int p = rb[s]; p < re[s]; ++p) {
                     double a[3];
-                    a[0] = minimg(sp[(size_t)p * PS] - x0, g.L[0], g.invL[0], g.periodic[0]);
-                    a[1] = minimg(sp[(size_t)p * PS + 1] - x1, g.L[1], g.invL[1], g.periodic[1]);
-                    a[2] = (D == 3) ? minimg(sp[(size_t)p * PS + 2] - x2, g.L[2], g.invL[2], g.periodic[2]) : 0.0;
+                    a[0] = minimg(sp[(size_t)p * PS] - x0, g.L[0], g.pinvL[0]);
+                    a[1] = minimg(sp[(size_t)p * PS + 1] - x1, g.L[1], g.pinvL[1]);
+                    a[2] = (D == 3) ? minimg(sp[(size_t)p * PS + 2] - x2, g.L[2], g.pinvL[2]) : 0.0;
                     const double asq = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
                     if (!(asq < g.rc2)) continue;
                     ++nball;

@@ -60,7 +60,7 @@ void launch_gather_f(const LaunchCtx &c, unsigned fields)
 template <int D, int OD, bool VIR, bool ELA, bool KIN, typename ER>
 void launch_tile_t(const LaunchCtx &c)
 {
-    using CP = TileCaps<D, OD, ELA, ER>;
+    using CP = TileCaps;
     const dim3 grid((unsigned)((c.G + CP::TP - 1) / CP::TP), (unsigned)c.nf);
     c.pl_launched = 1;
     hipLaunchKernelGGL((k_gather_tile<D, OD, 0, 5, VIR, ELA, KIN, ER>), grid, dim3(CP::BLK), 0, c.stream, c.g, c.n, c.G,

@@ -11,6 +11,7 @@ namespace pgt {
 // Geometry shared by all kernels of one launch.
 struct Geom {
     double L[3], invL[3], lo[3];
+    double pinvL[3];  // invL on periodic axes, 0 on the others
     int periodic[3];
     int nc[3];     // cells per axis: 1 (non-periodic axes, short boxes) or at least 2 reach + 1
     double cs[3];  // cell edge, at least (search radius) / reach
@@ -25,7 +26,7 @@ struct Geom {
     // and |a|^2 > out32 outside, whatever the FP64 rounding; in between the
     // FP64 test decides (in32 < 0 and out32 = inf send every candidate to
     // it).  bin32 and bout32 do the same for the bond search radius.
-    float L32[3], invL32[3];
+    float L32[3], invL32[3], pinvL32[3];
     float in32, out32;
     float bin32, bout32;
 };
@@ -33,11 +34,10 @@ struct Geom {
 // Minimum image with the image index rint(d/L) taken as rint(d * (1/L)).
 // The two agree except when d/L is within a few ulp of a half-integer, i.e.
 // for separations of half a box, which every search here rejects (search
-// radii are required to be below half a periodic box).
-__device__ __forceinline__ double minimg(double d, double L, double invL, int periodic)
-{
-    return periodic ? fma(-L, rint(d * invL), d) : d;
-}
+// radii are required to be below half a periodic box).  pinvL is 1/L on a
+// periodic axis and 0 on another, where the expression is exactly d (the
+// image index is rint(0) = 0 and fma(-L, 0, d) = d), so no select is needed.
+__device__ __forceinline__ double minimg(double d, double L, double pinvL) { return fma(-L, rint(d * pinvL), d); }
 
 __device__ __forceinline__ int cell_coord(const Geom &g, int a, double p)
 {

@@ -44,8 +44,10 @@ inline int gather_pl(int tuned, int cg, int nq, int D, int OD)
 inline bool tile_form(int cg, int nq, int D, int OD) { return cg == 0 && nq == 5 && !(D == 3 && OD == 2); }
 
 // The tile gather's launch shape: kTilePoints grid points per workgroup of
-// kTileThreads lanes (two per point), in every form.
-constexpr int kTilePoints = 128, kTileThreads = 256;
+// kTileThreads lanes (one wavefront, one point per lane), in every form.
+constexpr int kTilePoints = 64, kTileThreads = 64;
+// Particles per frame the tile gather can index (26 bits of an item word).
+constexpr int64_t kTileMaxParticles = (int64_t)1 << 26;
 
 // Field gather of one batch (fixed-order rule), one function per form and
 // translation unit: 2D positions with FP64 or FP32 bond entries, 3D positions

@@ -271,8 +271,8 @@ __device__ __forceinline__ void for_ball(const Geom &g, const int *__restrict__ 
         const int nr = row_ranges(g, st, rows[r], cx, rb, re);
         for (int s = 0; s < nr; ++s)
             for (int p = rb[s]; p < re[s]; ++p) {
-                const double a0 = minimg(sp[(size_t)p * 2] - x0, g.L[0], g.invL[0], g.periodic[0]);
-                const double a1 = minimg(sp[(size_t)p * 2 + 1] - x1, g.L[1], g.invL[1], g.periodic[1]);
+                const double a0 = minimg(sp[(size_t)p * 2] - x0, g.L[0], g.pinvL[0]);
+                const double a1 = minimg(sp[(size_t)p * 2 + 1] - x1, g.L[1], g.pinvL[1]);
                 const double asq = a0 * a0 + a1 * a1;
                 if (asq < g.rc2) visit(p, a0, a1, asq * g.inv_rc2);
             }

@@ -354,8 +354,9 @@ class Engine(object):
                "pgt_ik_set_tuning")
 
     def set_gather_mode(self, mode):
-        """'tile' (LDS-staged workgroup tiles) or 'lane' (one lane per grid
-        point from global memory) for the Octic 5-node rule."""
+        """'tile' (one wavefront per 64-point tile, work items balanced by
+        bond count) or 'lane' (one lane per grid point or group of points)
+        for the Octic 5-node rule; other rules always run the lane gather."""
         _check(lib().pgt_ik_set_gather_mode(self.h, {"lane": 0, "tile": 1}[mode]), "pgt_ik_set_gather_mode")
 
     def stats(self):

@@ -190,3 +190,32 @@ def test_tile_matches_lane_on_c2_frames(gpu):
     assert np.abs(res["tile"][1] - res["lane"][1]).max() <= 1e-13 * np.abs(res["lane"][1]).max()
     assert res["tile"][2]["n_contrib"] == res["lane"][2]["n_contrib"]
     assert res["tile"][2]["n_ball"] == res["lane"][2]["n_ball"]
+
+
+def test_tile_results_repeat_bit_for_bit(gpu):
+    """The tile gather's sums have a fixed order: two runs of the same job
+    give the same bits (2D FP64 virial + kinetic + elastic, 3D FP32)."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    for dim, prec, fields, N, ng in ((2, "fp64", ("virial", "kinetic", "elastic"), 9000, 64),
+                                     (3, "fp32", ("virial",), 6000, 14)):
+        frames = [workload.make_system(N, dim, 8700 + f) for f in range(2)]
+        L = frames[0]["L"]
+        grid = workload.cartesian_grid_2d(L[0], ng) if dim == 2 else workload.cartesian_grid_3d(L[0], ng)
+        e = _engine(dim, max(1.25 * 1.62, 1.62), precision=prec)
+        e.set_box(L, [1, 1, 1 if dim == 3 else 0])
+        e.set_grid(grid)
+        e.set_tuning(frames_per_launch=2)
+        t = lambda k: torch.tensor(np.stack([f[k] for f in frames]), dtype=torch.float64, device=dev)
+        disp = torch.tensor(np.random.default_rng(3).normal(0, 0.03, (2, N, 3)) * np.array([1, 1, 0 if dim == 2 else 1]),
+                            device=dev) if "elastic" in fields else None
+        outs = []
+        for rep in range(2):
+            o = {k: torch.zeros((2, len(grid), dim, dim), dtype=torch.float64, device=dev) for k in fields}
+            e.compute_frames(2, N, t("pos"), t("diam"), t("mass"), t("vel"), disp, fields=fields,
+                             d_virial=o.get("virial"), d_kinetic=o.get("kinetic"), d_elastic=o.get("elastic"))
+            torch.cuda.synchronize()
+            outs.append(o)
+        for k in fields:
+            assert torch.equal(outs[0][k], outs[1][k]), (dim, prec, k)

@@ -279,33 +279,45 @@ def test_interrupt_completes_the_frame_then_stops(tmp_path):
 
 
 def test_autotuner_picks_fastest_shape_and_caches(tmp_path):
-    """The autotuner times every (cell reach, points per lane, threads, frames
-    per launch) shape on the caller's job, keeps the fastest, and replays it from its
-    cache (in memory and on disk) without timing again."""
+    """The autotuner times every (gather kind, cell reach, points per lane,
+    threads, frames per launch) shape on the caller's job -- the tile gather
+    only over reach and frames per launch, its workgroup shape being fixed --
+    keeps the fastest, and replays it from its cache (in memory and on disk)
+    without timing again."""
     from pyglasstools_amd.autotune import Autotuner
 
     class FakeEngine(object):
         def __init__(self):
             self.calls = []
             self.shape = None
+            self.mode = "lane"
 
         def set_tuning(self, threads_per_block=0, frames_per_launch=0, points_per_lane=0, cell_reach=0):
-            self.calls.append((threads_per_block, frames_per_launch, points_per_lane, cell_reach))
+            self.calls.append((self.mode, threads_per_block, frames_per_launch, points_per_lane, cell_reach))
             self.shape = (threads_per_block, frames_per_launch, points_per_lane, cell_reach)
 
-    eng = FakeEngine()
-    cost = {(128, 10, 2, 2): 1e-4}
+        def set_gather_mode(self, mode):
+            self.mode = mode
 
-    def job():
-        import time
-        time.sleep(cost.get(eng.shape, 2e-3))
+    def run(cost, fast):
+        eng = FakeEngine()
 
-    path = str(tmp_path / "tune.json")
-    best, trials = Autotuner(path).tune(eng, job, ("k", 1), 40, fpl=(10, 100), repeats=1)
-    assert best == {"threads_per_block": 128, "frames_per_launch": 10, "points_per_lane": 2, "cell_reach": 2}
-    # 2 reaches x 3 points-per-lane x 2 blocks x 2 frame counts (100 is capped at the 40 frames)
-    assert len(trials) == 24 and {t["frames_per_launch"] for t in trials} == {10, 40}
-    assert eng.shape == (128, 10, 2, 2)
-    eng2 = FakeEngine()
-    best2, trials2 = Autotuner(path).tune(eng2, job, ("k", 1), 40)
-    assert best2 == best and trials2 == [] and eng2.calls == [(128, 10, 2, 2)]
+        def job():
+            import time
+            time.sleep(cost.get((eng.mode,) + eng.shape, 2e-3))
+
+        path = str(tmp_path / ("tune_%s.json" % fast[0]))
+        best, trials = Autotuner(path).tune(eng, job, ("k", 1), 40, fpl=(10, 100), repeats=1)
+        assert (best["gather"], best["threads_per_block"], best["frames_per_launch"], best["points_per_lane"],
+                best["cell_reach"]) == fast
+        # lane: 2 reaches x 3 points-per-lane x 2 blocks x 2 frame counts (100 is
+        # capped at the 40 frames); tile: 2 reaches x 2 frame counts
+        assert len(trials) == 28 and {t["frames_per_launch"] for t in trials} == {10, 40}
+        assert sum(t["gather"] == "tile" for t in trials) == 4
+        assert (eng.mode,) + eng.shape == fast
+        eng2 = FakeEngine()
+        best2, trials2 = Autotuner(path).tune(eng2, job, ("k", 1), 40)
+        assert best2 == best and trials2 == [] and eng2.calls == [("lane",) + fast[1:]] and eng2.mode == fast[0]
+
+    run({("lane", 128, 10, 2, 2): 1e-4}, ("lane", 128, 10, 2, 2))
+    run({("tile", 128, 40, 1, 1): 1e-4}, ("tile", 128, 40, 1, 1))
