@@ -220,12 +220,14 @@ int pgt_ik_set_tuning(pgt_ik_engine *e, int threads_per_block, int frames_per_la
                       int cell_reach);
 
 /* Field-gather kernel for the Octic 5-node rule (the driver scripts' rule):
- * 1, the tile gather (a 128-lane workgroup stages the particles and bond
- * entries of its window of cells in LDS once and every lane integrates from
- * there); 0, the lane gather (each lane reads particles and entries from
- * global memory).  Same results to rounding; other rules always use the lane
- * gather, and a tile whose window overflows its LDS arrays makes the job run
- * again with the lane gather.  No counterpart in the reference. */
+ * 1, the tile gather (one wavefront per 64 grid points lists the (point,
+ * particle) pairs of its balls, sorts them by bond count and integrates
+ * them round-robin, so every lane has about the same number of bonds);
+ * 0, the lane gather (each lane walks the balls of its own one, two or four
+ * points).  Same results to rounding; other rules always use the lane
+ * gather, as do frames of more than 2^26 particles, and a particle with more
+ * than 255 bonds makes the job run again with the lane gather.  No
+ * counterpart in the reference. */
 int pgt_ik_set_gather_mode(pgt_ik_engine *e, int mode);
 
 int pgt_ik_get_stats(pgt_ik_engine *e, pgt_ik_stats *out);

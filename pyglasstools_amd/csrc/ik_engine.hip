@@ -837,8 +837,8 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
         if (grow && mx > 65535) return fail(PGT_ERR_RANGE, "a particle has more than 65535 interacting neighbours");
         if (grow && e->held) return fail_held();  // the graph's launches keep the old capacity
         if (flags[1] & 6) {
-            // a tile window held more particles or entries than the LDS
-            // arrays: run the job again with the lane gather
+            // a tile item had more bonds than its one-byte count holds:
+            // run the job again with the lane gather
             if (!j.tile) return fail(PGT_ERR_STATE, "tile overflow reported by a lane gather");
             j = job_shape(e, fields, fs.z_flat, fs.n, false);
         } else if (!grow) {
@@ -1223,7 +1223,7 @@ int pgt_ik_finish(pgt_ik_engine *e)
         return fail(PGT_ERR_RANGE, "an adaptive bond integral needed more than " + std::to_string(kQagMax) +
                                        " intervals (the GPU rule's workspace)");
     if (flags[1] & 6)
-        return fail(PGT_ERR_RANGE, "a tile of the gather held more particles or bond entries than its LDS arrays: "
+        return fail(PGT_ERR_RANGE, "a particle had more than 255 bonds, more than the tile gather holds: "
                                    "the enqueued results are incomplete; select the lane gather "
                                    "(pgt_ik_set_gather_mode) and capture again");
     if (flags[0] > e->cap)

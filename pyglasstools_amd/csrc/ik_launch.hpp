@@ -28,8 +28,8 @@ struct LaunchCtx {
     const float4 *rec;  // FP32 position and bond count per sorted particle
     const void *bonds;
     GatherOut out;
-    int *flags;  // job flags: 1 an adaptive integral ran out of workspace, 2 / 4 a tile window overflowed LDS
-    int tile;    // 1: LDS-staged tile gather (Octic 5-node rule), 0: one lane per point from global memory
+    int *flags;  // job flags: 1 an adaptive integral ran out of workspace, 2 a tile item had more than 255 bonds
+    int tile;    // 1: item-balanced tile gather (Octic 5-node rule), 0: lane gather
     hipStream_t stream;
 };
 
@@ -40,7 +40,7 @@ inline int gather_pl(int tuned, int cg, int nq, int D, int OD)
     return (cg == 0 && nq == 5 && !(D == 3 && OD == 2)) ? tuned : 1;
 }
 
-// Whether the LDS-staged tile gather serves this form (the Octic 5-node rule).
+// Whether the tile gather serves this form (the Octic 5-node rule).
 inline bool tile_form(int cg, int nq, int D, int OD) { return cg == 0 && nq == 5 && !(D == 3 && OD == 2); }
 
 // The tile gather's launch shape: kTilePoints grid points per workgroup of

@@ -1,8 +1,9 @@
-"""The LDS-staged tile gather (pgt_ik_set_gather_mode 1) against the oracle
+"""The item-balanced tile gather (pgt_ik_set_gather_mode 1) against the oracle
 and against the lane gather, through the C ABI: 2D and 3D, FP64 and the FP32
 form, virial / kinetic / elastic fields, the fixture trajectory, windows that
-wrap a whole periodic axis, 3D windows processed in several LDS chunks, a
-non-periodic axis, cell reach 2, and several frames per launch."""
+wrap a whole periodic axis, 3D tiles whose item lists are processed in
+several pieces, a non-periodic axis, cell reach 2, several frames per launch,
+and bit-for-bit repeatability."""
 import numpy as np
 import pytest
 
@@ -219,3 +220,25 @@ def test_tile_results_repeat_bit_for_bit(gpu):
             outs.append(o)
         for k in fields:
             assert torch.equal(outs[0][k], outs[1][k]), (dim, prec, k)
+
+
+def test_tile_more_than_255_bonds_runs_the_lane_gather(gpu):
+    """A particle with more than 255 bonds does not fit the tile gather's
+    one-byte bond count: the tile reports it and the job runs again with the
+    lane gather, which still matches the oracle."""
+    N, rc = 4096, 10.0
+    sysd = workload.make_system(N, 2, 8200)
+    grid = workload.cartesian_grid_2d(sysd["L"][0], 16)
+    d = float(sysd["diam"].max())
+    mr = rc * d
+    params = oracle.poly12_params(rcut=rc)
+    print("tile 2d N=%d with pair cutoff %.1f sigma" % (N, rc), flush=True)
+    e = engine.Engine(dim=2, params=params, scaled_rcut=rc, max_rcut=mr)
+    e.set_gather_mode("tile")
+    e.set_system(sysd, grid)
+    got = e.compute(fields=("virial",))
+    st = e.stats()
+    assert st["max_bonds"] > 255
+    ref = oracle.ik_compute(sysd, grid, params=params, scaled_rcut=rc, fields=("virial",), max_rcut=mr,
+                            with_abs=True)
+    _check(got, ref, ("virial",), st)
