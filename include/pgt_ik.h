@@ -112,6 +112,16 @@ const char *pgt_last_error(void);
 /* Number of visible HIP devices (0 without a GPU; never fails). */
 int pgt_device_count(void);
 
+/* Thresholds of the FP32 distance pre-test of the 3D candidate scans for a
+ * search radius r, as the engine sets them: coordinates (box and grid) at
+ * most max_coord in magnitude, box lengths max_box and min_box over the
+ * searched axes, periodic = every searched axis periodic.  FP32 |a|^2 below
+ * *in32 is inside r in FP64 and above *out32 outside; without a bound
+ * *in32 = -1 and *out32 = inf.  Host code only (no device needed); for
+ * checking the bound.  No counterpart in the reference. */
+int pgt_pretest_band(double r, double max_coord, double max_box, double min_box, int dim, int periodic,
+                     float *in32, float *out32);
+
 /* Create an engine on the current HIP device.
  * Replaces: IrvingKirkwood(ParticleSystem, PairPotential, CoarseGrainFunction,
  * Communicator), irvingkirkwood/IrvingKirkwood.h:12-18, 116. */
@@ -268,6 +278,12 @@ int pgt_ik_bond_samples(pgt_ik_engine *e, int64_t max_pairs, float *d_x, float *
  * integral from the network where the error bound of the pair's class (both
  * ends inside the ball: bound_inside; the segment leaves it: bound_crossing)
  * is at most tol, and from the 5-node rule elsewhere (tol < 0: everywhere).
+ * tol and the bounds are absolute errors of one unnormalised bond integral,
+ * not a tolerance on the field; the bounds are the caller's (the package
+ * calibrates them empirically, as twice the largest error seen on held-out
+ * frames), not guarantees.  When neither bound is within tol no pair can
+ * take the network's value, and the field comes from the plain gather
+ * (the pair list is skipped; n_pairs is then its contribution count).
  * Replaces IrvingKirkwood::compute (IrvingKirkwood.h:74-110) for the virial
  * field in this form; counts in pgt_ik_get_stats (n_pairs, n_network). */
 int pgt_ik_compute_surrogate(pgt_ik_engine *e, const float *d_params, double bound_inside, double bound_crossing,

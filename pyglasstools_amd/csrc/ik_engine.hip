@@ -32,6 +32,7 @@
 #include <string>
 #include <vector>
 
+#include "ik_band.hpp"
 #include "ik_kernels.hpp"
 #include "ik_launch.hpp"
 #include "pgt_ik.h"
@@ -426,15 +427,10 @@ Geom make_geom(const pgt_ik_engine *e, int D)
     g.inv_rc = 1.0 / e->prm.cg_rcut;
     g.inv_rc2 = 1.0 / g.rc2;
     g.mr2 = e->prm.max_rcut * e->prm.max_rcut;
-    // FP32 pre-test bounds for a search radius r (r_cg for the gather,
-    // max_rcut for the bond list); every axis searched must be periodic, so
-    // that particle coordinates are wrapped into the box.  Coordinates (box
-    // and grid) are at most M in magnitude; an FP32 component of a
-    // separation (difference, minimum image) is then within
-    // E = (4M + 2L + 4r) 2^-23 of the FP64 one (twice the rounding bound),
-    // and near |a| = r the FP32 |a|^2 is within D E (2.02 r + 2E) +
-    // 3 2^-24 |a|^2 of the FP64 value; the band is twice that, and the
-    // thresholds are rounded outwards.
+    // FP32 pre-test bounds for the two search radii (r_cg for the gather,
+    // max_rcut for the bond list): fp32_band, with M the largest coordinate
+    // magnitude of box and grid; every searched axis must be periodic, so
+    // that particle coordinates are wrapped into the box
     for (int a = 0; a < 3; ++a) {
         g.L32[a] = (float)g.L[a];
         g.invL32[a] = (float)g.invL[a];
@@ -448,23 +444,8 @@ Geom make_geom(const pgt_ik_engine *e, int D)
         Lm = std::max(Lm, e->L[a]);
         Lmin = std::min(Lmin, e->L[a]);
     }
-    // (a search radius well below half of every box length, so that the FP32
-    // and FP64 minimum images can differ only for candidates far outside)
-    auto band = [&](double r, float &in32, float &out32) {
-        in32 = -1.0f;
-        out32 = INFINITY;
-        const double r2 = r * r;
-        const double Ee = (4 * M + 2 * Lm + 4 * r) * std::ldexp(1.0, -23);
-        const double B = 2 * (D * Ee * (2.02 * r + 2 * Ee) + 3 * std::ldexp(1.0, -24) * 1.03 * r2);
-        if (!(ok && r2 - B > 0 && 2 * r + 8 * Ee < Lmin)) return;
-        float lo32 = (float)(r2 - B), hi32 = (float)(r2 + B);
-        if ((double)lo32 > r2 - B) lo32 = std::nextafter(lo32, -INFINITY);
-        if ((double)hi32 < r2 + B) hi32 = std::nextafter(hi32, INFINITY);
-        in32 = lo32;
-        out32 = hi32;
-    };
-    band(e->prm.cg_rcut, g.in32, g.out32);
-    band(e->prm.max_rcut, g.bin32, g.bout32);
+    fp32_band(e->prm.cg_rcut, M, Lm, Lmin, D, ok, g.in32, g.out32);
+    fp32_band(e->prm.max_rcut, M, Lm, Lmin, D, ok, g.bin32, g.bout32);
     return g;
 }
 
@@ -984,6 +965,15 @@ unsigned blocks_for(int64_t threads) { return (unsigned)((threads + 255) / 256);
 extern "C" {
 
 const char *pgt_last_error(void) { return g_err.c_str(); }
+
+int pgt_pretest_band(double r, double max_coord, double max_box, double min_box, int dim, int periodic,
+                     float *in32, float *out32)
+{
+    if (!in32 || !out32) return fail(PGT_ERR_ARG, "null output");
+    if (dim != 2 && dim != 3) return fail(PGT_ERR_ARG, "dim must be 2 or 3");
+    fp32_band(r, max_coord, max_box, min_box, dim, periodic != 0, *in32, *out32);
+    return PGT_OK;
+}
 
 int pgt_device_count(void)
 {
@@ -1682,6 +1672,15 @@ int pgt_ik_compute_surrogate(pgt_ik_engine *e, const float *d_params, double bou
         return PGT_OK;
     }
     int rc;
+    // no pair class qualifies for the network: the field is the plain form's,
+    // so run the plain gather (the pair list would only cost time)
+    if (!(tol >= 0.0 && std::min(bound_inside, bound_crossing) <= tol)) {
+        if ((rc = pgt_ik_compute(e, PGT_FIELD_VIRIAL, virial, nullptr, nullptr))) return rc;
+        e->stats.n_pairs = e->stats.n_contrib;
+        e->stats.n_network = 0;
+        e->stats.surrogate_ms = 0.0;
+        return PGT_OK;
+    }
     // events 0..2 belong to the bond-list job inside prepare_pairs
     if ((rc = ensure_events(e, 6))) return rc;
     if ((rc = e->d_out.ensure((size_t)G * 4)) || (rc = e->d_pn.ensure(2))) return rc;

@@ -27,7 +27,7 @@ EXPORTED = [
     "pgt_ik_finish", "pgt_ik_release_enqueued", "pgt_ik_set_stream",
     "pgt_ik_set_tuning", "pgt_ik_set_gather_mode", "pgt_ik_get_stats", "pgt_eval_pair_force", "pgt_eval_cgfunc",
     "pgt_mlp_nparam", "pgt_mlp_work_floats", "pgt_mlp_forward", "pgt_mlp_grad", "pgt_mlp_adam", "pgt_mlp_infer",
-    "pgt_ik_bond_samples", "pgt_ik_compute_surrogate",
+    "pgt_ik_bond_samples", "pgt_ik_compute_surrogate", "pgt_pretest_band",
 ]
 
 
@@ -81,6 +81,8 @@ def lib():
         vp, i, d, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
         L.pgt_last_error.restype = ctypes.c_char_p
         L.pgt_device_count.restype = i
+        L.pgt_pretest_band.argtypes = [d, d, d, d, i, i, ctypes.POINTER(ctypes.c_float),
+                                       ctypes.POINTER(ctypes.c_float)]
         L.pgt_ik_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(vp)]
         L.pgt_ik_destroy.argtypes = [vp]
         L.pgt_ik_destroy.restype = None
@@ -169,6 +171,15 @@ def _f64(a, shape=None):
 
 def device_count():
     return lib().pgt_device_count()
+
+
+def pretest_band(r, max_coord, max_box, min_box, dim=3, periodic=True):
+    """(in32, out32): the FP32 pre-test thresholds the engine sets for search
+    radius r (pgt_pretest_band); host code, no device needed."""
+    lo, hi = ctypes.c_float(), ctypes.c_float()
+    _check(lib().pgt_pretest_band(float(r), float(max_coord), float(max_box), float(min_box), int(dim),
+                                  1 if periodic else 0, ctypes.byref(lo), ctypes.byref(hi)), "pgt_pretest_band")
+    return np.float32(lo.value), np.float32(hi.value)
 
 
 def poly12_params(v0=1.0, eps=0.2, rcut=1.25):

@@ -12,12 +12,15 @@ the engine's own kernels:
   (`pgt_mlp_grad`, `pgt_mlp_adam`), on shuffled minibatches;
 * calibration: the largest |network - quadrature| over held-out samples, per
   pair class (both bond ends inside the coarse-graining ball, or not), times a
-  safety factor, is the class's error bound;
+  safety factor, is the class's error bound -- an empirical bound on a single
+  bond integral, not a guarantee and not a tolerance on the field;
 * the field form (`Engine.compute_surrogate`) takes the network's value for a
   pair whose class bound is within the tolerance and the quadrature's
   otherwise.
 
-Torch provides device memory and the stream only.
+Torch provides device memory and the stream, and its stock kernels do the
+bookkeeping around the engine's: the minibatch shuffle and row gather in
+training, and the error reductions of calibration.
 """
 import time
 

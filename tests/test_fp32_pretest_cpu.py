@@ -3,7 +3,8 @@
 The 3D gather and bond kernels form |a|^2 from FP32 copies of the
 coordinates (difference, minimum image by fmaf(-L, rintf(d / L), d), sum of
 squares) and compare it with two FP32 thresholds set by the engine's host
-code (make_geom in ik_engine.hip): below in32 the candidate is taken as
+code (fp32_band in ik_band.hpp, exported as pgt_pretest_band and checked
+bit for bit against the restatement here): below in32 the candidate is taken as
 inside the search radius r, above out32 as outside, and only in between is
 the FP64 test run.  The thresholds are r^2 -+ B with
     E = (4M + 2L + 4r) 2^-23,  B = 2 (D E (2.02 r + 2E) + 3 2^-24 1.03 r^2),
@@ -112,3 +113,38 @@ def test_band_is_narrow_at_the_benchmark_sizes():
     in32, out32, B = thresholds(2.0, L / 2, L)
     assert B / 4.0 < 2e-4
     assert float(out32) - float(in32) < 2e-3
+
+
+def _restated_band(r, M, Lmax, Lmin, D, ok):
+    """The thresholds with the engine's gating: (-1, inf) without a bound."""
+    lo, hi, B = thresholds(r, M, Lmax, D)
+    E = (4 * M + 2 * Lmax + 4 * r) * 2.0 ** -23
+    if not (ok and r * r - B > 0 and 2 * r + 8 * E < Lmin):
+        return F32(-1.0), F32(np.inf)
+    return lo, hi
+
+
+def test_engine_band_is_the_restated_band_bit_for_bit():
+    """The thresholds the engine's host code sets (pgt_pretest_band, the
+    function make_geom calls) equal the restatement whose soundness the test
+    above checks, bit for bit, over radii, coordinate ranges, box shapes and
+    both dimensions, including the cases without a bound (an open axis, a
+    radius near half the box, a band reaching zero)."""
+    from pyglasstools_amd import engine
+
+    rng = np.random.default_rng(11)
+    cases = [(2.0, 25.4, 50.8, 50.8, 3, True), (2.0, 25.4, 50.8, 50.8, 2, True), (1.2, 1.6, 3.2, 3.2, 3, True),
+             (2.0, 25.4, 50.8, 50.8, 3, False), (1.6, 1.7, 3.3, 3.2, 3, True), (1e-3, 1e6, 2e6, 2e6, 3, True)]
+    for _ in range(300):
+        L = rng.uniform(3.0, 500.0, 3)
+        far = rng.choice([0.0, rng.uniform(0.0, 10.0)])
+        M = L.max() / 2 * (1 + 2 * far)
+        cases.append((rng.uniform(0.05, 0.45) * L.min(), M, L.max(), L.min(), int(rng.choice([2, 3])), True))
+    for r, M, Lmax, Lmin, D, ok in cases:
+        got = engine.pretest_band(r, M, Lmax, Lmin, D, ok)
+        want = _restated_band(r, M, Lmax, Lmin, D, ok)
+        assert np.float32(got[0]).tobytes() == np.float32(want[0]).tobytes(), (r, M, Lmax, Lmin, D, ok, got, want)
+        assert np.float32(got[1]).tobytes() == np.float32(want[1]).tobytes(), (r, M, Lmax, Lmin, D, ok, got, want)
+    # the gating cases do occur among the fixed ones
+    assert engine.pretest_band(2.0, 25.4, 50.8, 50.8, 3, False)[0] == -1.0
+    assert engine.pretest_band(1.6, 1.7, 3.3, 3.2, 3, True)[0] == -1.0
