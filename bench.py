@@ -430,6 +430,9 @@ def learned_surrogate(args):
     print(json.dumps(line), flush=True)
 
 
+C5_N = (4096, 8192, 16384, 24576, 32768, 49152, 65536, 98304)
+
+
 def batched_sweep(args):
     """Serving-style sweep: 1000 2D frames from 8 systems of uneven N (125
     frames each, seeds 5000 + 1000 k + f), each on a 128 x 128 grid over its

@@ -1,7 +1,8 @@
 """End to end on the GPU: the pyglasstools IK driver sequence (potential,
-coarse-graining function, GSD ingest, ikcalculator, fieldlogger, analyze)
-against the polydisperse12.gsd fixture, through the pybind11 surface; the
-.xyz output is checked against the oracle's fields."""
+coarse-graining function, GSD ingest, ikcalculator or radialcalculator,
+fieldlogger, analyze) against the polydisperse12.gsd fixture, through the
+pybind11 surface; the .xyz output (cartesian and polar rows) is checked
+against the oracle's fields."""
 import subprocess
 import sys
 import textwrap
@@ -71,6 +72,69 @@ def test_driver_sequence_writes_oracle_fields(gpu, tmp_path):
                 assert v == pytest.approx(float("%g" % w), rel=2e-6, abs=1e-300) or abs(v - w) <= 5e-6 * abs(w)
         k += 2 + G
     assert open(tmp_path / "restart.log").read() == "Frame 183"
+    traj.close()
+
+
+POLAR_SCRIPT = textwrap.dedent("""
+    import sys
+    sys.path.insert(0, {root!r})
+    import numpy as np
+    import pyglasstools as pglass
+    import pyglasstools.io as io
+    import pyglasstools.utils as utils
+    import pyglasstools.potential as pot
+    import pyglasstools.irvingkirkwood as ik
+    import pyglasstools.irvingkirkwood.cgfunc as cgfunc
+    poly12 = pot.polydisperse12(v0=1.0, eps=0.2, rcut=1.25)
+    mycgfunc = cgfunc.octic(rcut=2.0, mode="fixedpoint", order=5)
+    mysystem = utils.read_gsd(filename={gsd!r}, checkpointfile="restart.log")
+    MyCalculator = ik.radialcalculator(cgfunc=mycgfunc, center=np.array([1.5, -2.0, 0.0]), rmax=6, dr=0.5)
+    names = ['virialstress_00', 'virialstress_11', 'virialstress_01', 'kineticstress_00']
+    mylogger = io.fieldlogger(keyword="polar-dump", names=names, solver=MyCalculator)
+    pglass.analyze(frame_list=range(180, 182))
+""")
+
+
+def test_radialcalculator_polar_output_matches_oracle(gpu, tmp_path):
+    """The driver sequence with radialcalculator (irvingkirkwood/__init__.py:
+    80-166) on an off-centre polar grid: the .xyz rows carry the polar prefix
+    (1, ring id, r, theta; io.py:130-145) and the GPU fields, which match the
+    oracle on the same points."""
+    (tmp_path / "drive.py").write_text(POLAR_SCRIPT.format(root=ROOT, gsd=FIXTURE_GSD))
+    r = subprocess.run([sys.executable, "drive.py"], cwd=str(tmp_path), capture_output=True, text=True, timeout=600)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    from pyglasstools_amd.gsd_reader import open as gsd_open
+    from pyglasstools_amd.irvingkirkwood import polar_grid
+
+    gid, gr, gth, grid = polar_grid(np.array([1.5, -2.0, 0.0]), "normal", 6, 0, 0.5, 0.05)
+    G = len(grid)
+    assert G > 100
+    lines = open(tmp_path / "polar-dump.xyz").read().splitlines()
+    traj = gsd_open(FIXTURE_GSD)
+    f0 = traj[180]
+    L = float(f0.configuration.box[0])
+    d0 = float(f0.particles.diameter.astype(np.float64).max())
+    max_rcut = max(1.25 * d0, d0)
+    k = 0
+    for frame in (180, 181):
+        assert lines[k] == "%d " % G and lines[k + 1] == "#Frame %d  " % frame
+        fr = traj[frame]
+        sysd = {"dim": 2, "L": np.array([L, L, 1.0]), "periodic": np.array([1, 1, 0]),
+                "pos": fr.particles.position.astype(np.float64), "diam": fr.particles.diameter.astype(np.float64),
+                "mass": fr.particles.mass.astype(np.float64), "vel": fr.particles.velocity.astype(np.float64)}
+        ref = oracle.ik_compute(sysd, grid, max_rcut=max_rcut, fields=("virial", "kinetic"))
+        for g in range(G):
+            cols = lines[k + 2 + g].split(" ")
+            assert cols[0] == "1" and int(cols[1]) == int(gid[g])
+            assert float(cols[2]) == gr[g] and float(cols[3]) == gth[g]
+            vals = [float(c) for c in cols[4:8]]
+            want = [ref["virial"][g, 0, 0], ref["virial"][g, 1, 1], ref["virial"][g, 1, 0], ref["kinetic"][g, 0, 0]]
+            for v, w in zip(vals, want):
+                # 6 significant digits in the file
+                assert v == pytest.approx(float("%g" % w), rel=2e-6, abs=1e-300) or abs(v - w) <= 5e-6 * abs(w)
+        k += 2 + G
+    assert len(lines) == k
     traj.close()
 
 
