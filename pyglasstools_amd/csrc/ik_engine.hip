@@ -221,30 +221,6 @@ struct DBuf {
     }
 };
 
-// Pinned host staging memory (hipHostMalloc), grown as needed.
-struct HostBuf {
-    unsigned char *p = nullptr;
-    size_t n = 0;
-    int ensure(size_t bytes)
-    {
-        if (bytes <= n && p) return 0;
-        release();
-        hipError_t e = hipHostMalloc((void **)&p, bytes ? bytes : 1, hipHostMallocDefault);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return fail(PGT_ERR_HIP, std::string("hipHostMalloc: ") + hipGetErrorString(e));
-        }
-        n = bytes ? bytes : 1;
-        return 0;
-    }
-    void release()
-    {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
-
 }  // namespace
 
 struct pgt_ik_engine {
@@ -270,7 +246,6 @@ struct pgt_ik_engine {
     DBuf<unsigned char> d_bonds;
     DBuf<int> d_maxcount;  // [0] largest bond count, [1] adaptive workspace ran out
     DBuf<double> d_out;
-    HostBuf h_stage;             // pinned staging of host-buffer uploads and downloads
     std::vector<double> h_grid;  // the current grid (set_grid skips an unchanged one)
     DBuf<double> d_gl;
     DBuf<unsigned long long> d_stats;              // job totals (balls, contributions)
@@ -1044,7 +1019,6 @@ void pgt_ik_destroy(pgt_ik_engine *e)
     e->d_pw.release();
     e->d_pn.release();
     e->d_stats.release();
-    e->h_stage.release();
     for (hipEvent_t v : e->ev) (void)hipEventDestroy(v);
     if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
     delete e;
@@ -1086,22 +1060,16 @@ int pgt_ik_set_particles(pgt_ik_engine *e, int64_t n, const double *pos, const d
     e->have_vel = vel != nullptr;
     e->have_disp = disp != nullptr;
     if (n > 0) {
-        // through pinned staging memory: one host copy, then DMA at full rate
+        // straight from the caller's (pageable) arrays: the runtime stages
+        // them through its own pinned buffers with the copies pipelined,
+        // which measured faster than one host copy into pinned memory of
+        // ours followed by the DMA
         const double *src[5] = {pos, diam, mass, vel, disp};
         double *dst[5] = {e->d_pos.p, e->d_diam.p, e->d_mass.p, e->d_vel.p, e->d_disp.p};
         const size_t cnt[5] = {3, 1, 1, 3, 3};
-        size_t tot = 0;
         for (int k = 0; k < 5; ++k)
-            if (src[k]) tot += cnt[k] * (size_t)n;
-        if ((rc = e->h_stage.ensure(tot * sizeof(double)))) return rc;
-        double *h = reinterpret_cast<double *>(e->h_stage.p);
-        for (int k = 0; k < 5; ++k) {
-            if (!src[k]) continue;
-            const size_t bytes = sizeof(double) * cnt[k] * (size_t)n;
-            std::memcpy(h, src[k], bytes);
-            HIPCHK(hipMemcpyAsync(dst[k], h, bytes, hipMemcpyHostToDevice, s));
-            h += cnt[k] * (size_t)n;
-        }
+            if (src[k])
+                HIPCHK(hipMemcpyAsync(dst[k], src[k], sizeof(double) * cnt[k] * (size_t)n, hipMemcpyHostToDevice, s));
     }
     bool flat = true;
     for (int64_t i = 0; i < n && flat; ++i) flat = pos[3 * i + 2] == 0.0;
@@ -1306,18 +1274,14 @@ int pgt_ik_compute(pgt_ik_engine *e, unsigned fields, double *virial, double *ki
     fs.ela = e->d_out.p + 2 * fsz;
     fs.z_flat = e->z_flat && e->grid_z_flat;
     if ((rc = run_job(e, fs, fields))) return rc;
-    // through pinned staging memory
-    if ((rc = e->h_stage.ensure(3 * fsz * sizeof(double)))) return rc;
-    double *h = reinterpret_cast<double *>(e->h_stage.p);
+    // straight into the caller's arrays (see pgt_ik_set_particles)
     const unsigned bits[3] = {PGT_FIELD_VIRIAL, PGT_FIELD_KINETIC, PGT_FIELD_ELASTIC};
     const double *dsrc[3] = {fs.vir, fs.kin, fs.ela};
     double *hdst[3] = {virial, kinetic, elastic};
     for (int k = 0; k < 3; ++k)
         if (fields & bits[k])
-            HIPCHK(hipMemcpyAsync(h + k * fsz, dsrc[k], fsz * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+            HIPCHK(hipMemcpyAsync(hdst[k], dsrc[k], fsz * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
-    for (int k = 0; k < 3; ++k)
-        if (fields & bits[k]) std::memcpy(hdst[k], h + k * fsz, fsz * sizeof(double));
     return PGT_OK;
 }
 
