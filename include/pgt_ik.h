@@ -82,8 +82,10 @@ typedef struct pgt_ik_params {
     double cg_rcut;           /* CoarseGrainFunction::cg_rcut */
     int quad_mode;            /* enum pgt_quad_mode */
     int quad_order;           /* fixed rule: number of nodes (1..1024) */
-    int adapt_limit;          /* adaptive rule: GSL workspace limit (neval); integrals needing more
-                                 than 128 intervals return PGT_ERR_RANGE */
+    int adapt_limit;          /* adaptive rule: GSL workspace limit (neval), honoured in full: integrals
+                                 needing more than 128 intervals get tables of this limit in global
+                                 memory (compute and compute_frames; the enqueue and no-allocation
+                                 entry points return PGT_ERR_RANGE instead) */
     double adapt_epsrel;
     double adapt_epsabs;
     int precision;            /* enum pgt_precision */

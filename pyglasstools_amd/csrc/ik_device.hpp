@@ -167,9 +167,34 @@ __device__ __forceinline__ ER bond_quad(const GLRule &R, const double *__restric
 // is at the rounding floor); otherwise bisect the interval with the largest
 // error estimate until the summed error meets the tolerance or `limit`
 // intervals exist, and return the sum of the interval results in storage
-// order.  Up to kQagMax intervals are kept per call; a call that would need
-// more (limit > kQagMax and not converged) sets `capped`.
+// order.  The interval table lives in a workspace W: QagPrivate keeps up to
+// kQagMax intervals in the lane's private memory, QagGlobal the caller's
+// limit in global memory; a call that would need more than its workspace
+// holds (limit larger and not converged) sets `capped`.
 constexpr int kQagMax = 128;
+
+// Interval table of kQagMax entries in private memory.
+struct QagPrivate {
+    double v[4][kQagMax];
+    __device__ int size() const { return kQagMax; }
+    __device__ double &a(int k) { return v[0][k]; }
+    __device__ double &b(int k) { return v[1][k]; }
+    __device__ double &r(int k) { return v[2][k]; }
+    __device__ double &e(int k) { return v[3][k]; }
+};
+
+// Interval table of n entries in global memory, lane-interleaved: entry k's
+// four values at p[(4k + q) * stride], p the lane's first slot.
+struct QagGlobal {
+    double *p;
+    size_t stride;
+    int n;
+    __device__ int size() const { return n; }
+    __device__ double &a(int k) { return p[(size_t)(4 * k) * stride]; }
+    __device__ double &b(int k) { return p[(size_t)(4 * k + 1) * stride]; }
+    __device__ double &r(int k) { return p[(size_t)(4 * k + 2) * stride]; }
+    __device__ double &e(int k) { return p[(size_t)(4 * k + 3) * stride]; }
+};
 
 struct AdaptQ {
     int limit;
@@ -252,18 +277,17 @@ __device__ void gk15_dev(const F &f, double a, double b, double &result, double 
     resasc = rasc;
 }
 
-template <typename F>
-__device__ double qag_dev(const F &f, const AdaptQ &Q, bool &capped)
+template <typename F, typename W>
+__device__ double qag_dev(const F &f, const AdaptQ &Q, bool &capped, W &w)
 {
-    double al[kQagMax], bl[kQagMax], rl[kQagMax], el[kQagMax];
     const int limit = Q.limit < 1 ? 1 : Q.limit;
-    const int lim = limit < kQagMax ? limit : kQagMax;
+    const int lim = limit < w.size() ? limit : w.size();
     double result, abserr, resabs, resasc;
     gk15_dev(f, 0.0, 1.0, result, abserr, resabs, resasc);
-    al[0] = 0.0;
-    bl[0] = 1.0;
-    rl[0] = result;
-    el[0] = abserr;
+    w.a(0) = 0.0;
+    w.b(0) = 1.0;
+    w.r(0) = result;
+    w.e(0) = abserr;
     int n = 1;
     double tol = fmax(Q.epsabs, Q.epsrel * fabs(result));
     const double round_off = 50 * 2.2204460492503131e-16 * resabs;
@@ -271,30 +295,36 @@ __device__ double qag_dev(const F &f, const AdaptQ &Q, bool &capped)
     double area = result, errsum = abserr;
     while (n < lim) {
         int imax = 0;
-        for (int k = 1; k < n; ++k)
-            if (el[k] > el[imax]) imax = k;
-        const double a1 = al[imax], b2 = bl[imax];
+        double emax = w.e(0);
+        for (int k = 1; k < n; ++k) {
+            const double ek = w.e(k);
+            if (ek > emax) {
+                emax = ek;
+                imax = k;
+            }
+        }
+        const double a1 = w.a(imax), b2 = w.b(imax);
         const double a2 = 0.5 * (a1 + b2), b1 = a2;
         double r1, e1, ra1, rs1, r2, e2, ra2, rs2;
         gk15_dev(f, a1, b1, r1, e1, ra1, rs1);
         gk15_dev(f, a2, b2, r2, e2, ra2, rs2);
-        area += r1 + r2 - rl[imax];
-        errsum += e1 + e2 - el[imax];
-        al[imax] = a1;
-        bl[imax] = b1;
-        rl[imax] = r1;
-        el[imax] = e1;
-        al[n] = a2;
-        bl[n] = b2;
-        rl[n] = r2;
-        el[n] = e2;
+        area += r1 + r2 - w.r(imax);
+        errsum += e1 + e2 - w.e(imax);
+        w.a(imax) = a1;
+        w.b(imax) = b1;
+        w.r(imax) = r1;
+        w.e(imax) = e1;
+        w.a(n) = a2;
+        w.b(n) = b2;
+        w.r(n) = r2;
+        w.e(n) = e2;
         ++n;
         tol = fmax(Q.epsabs, Q.epsrel * fabs(area));
         if (errsum <= tol) break;
     }
     if (n == lim && lim < limit && errsum > tol) capped = true;
     double sum = 0.0;
-    for (int k = 0; k < n; ++k) sum += rl[k];
+    for (int k = 0; k < n; ++k) sum += w.r(k);
     return sum;
 }
 

@@ -82,12 +82,17 @@ __global__ void k_eval_force(PotParams P, int n, const double *rij, const double
     }
 }
 
+// Device memory the adaptive rule may take for interval tables beyond the
+// kQagMax private ones (tables of the caller's limit, 32 bytes an interval,
+// one per lane): compute jobs fit their frames per launch to it.
+constexpr size_t kQagWorkspaceBytes = (size_t)4 << 30;
+
 // Kernel and bond integral in the reference's arithmetic form: the integrand
 // is Delta(|x - (ri + s rij)|^2) (CoarseGrainFunction.h:42-45), summed over
 // the unfolded Gauss-Legendre rule.
 __global__ void k_eval_cg(int cg, double rc, int mode, int n, const double *s, const double *x, const double *ri,
                           const double *rij, const double *gld, int nq, int adapt, AdaptQ Q, int *capped,
-                          double *out)
+                          double *ws, double *out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -103,8 +108,15 @@ __global__ void k_eval_cg(int cg, double rc, int mode, int n, const double *s, c
     if (mode == 0) out[i] = delta_at(0.0);
     else if (mode == 1) out[i] = delta_at(s[i]);
     else if (adapt) {
+        // interval tables of the caller's limit in ws when it exceeds kQagMax
         bool cp = false;
-        out[i] = qag_dev(delta_at, Q, cp);
+        if (ws) {
+            QagGlobal w{ws + i, (size_t)n, Q.limit};
+            out[i] = qag_dev(delta_at, Q, cp, w);
+        } else {
+            QagPrivate w;
+            out[i] = qag_dev(delta_at, Q, cp, w);
+        }
         if (cp) *capped = 1;
     } else {
         double acc = 0.0;
@@ -248,6 +260,7 @@ struct pgt_ik_engine {
     DBuf<double> d_out;
     std::vector<double> h_grid;  // the current grid (set_grid skips an unchanged one)
     DBuf<double> d_gl;
+    DBuf<double> d_qag;  // adaptive rule: interval tables beyond kQagMax intervals
     DBuf<unsigned long long> d_stats;              // job totals (balls, contributions)
     DBuf<unsigned char> d_wcounts, d_wmax, d_rec;  // per-wavefront counts and maxima; 3D FP32 records
     // surrogate field form: pair counts and offsets per grid point, pair
@@ -467,6 +480,7 @@ struct JobShape {
     int block, pl;
     bool tile;       // the item-balanced tile gather (kTilePoints points per workgroup of kTileThreads lanes)
     int64_t gwaves;  // gather wavefronts per frame
+    bool qag_global; // adaptive rule: interval tables of the caller's limit in e->d_qag
 };
 
 // n: particles per frame.  allow_tile false: the lane gather even where the
@@ -717,7 +731,14 @@ int run_frames(pgt_ik_engine *e, const JobShape &j, const Work &w, int cap, int 
         // cell list and bond list only (the surrogate field form reads them)
     } else if (j.adaptive) {
         const AdaptQ Q{e->prm.adapt_limit, e->prm.adapt_epsabs, e->prm.adapt_epsrel};
-        launch_gather_adaptive(c, fields, cg, D, OD, Q, c.flags);
+        if (j.qag_global) {
+            // sized by run_job for this batch's frames
+            if (e->d_qag.n < adaptive_workspace_doubles(c, Q.limit))
+                return fail(PGT_ERR_STATE, "adaptive workspace smaller than the batch needs");
+            launch_gather_adaptive(c, fields, cg, D, OD, Q, c.flags, e->d_qag.p, Q.limit);
+        } else {
+            launch_gather_adaptive(c, fields, cg, D, OD, Q, c.flags, nullptr, 0);
+        }
     } else if (D == 2) {
         if (j.fp32) launch_gather_2d32(c, fields, cg);
         else launch_gather_2d64(c, fields, cg);
@@ -764,15 +785,15 @@ int run_batch(pgt_ik_engine *e, const JobShape &j, const Work &w, int cap, const
 // capacity once; on overflow grow it and run the whole job again.
 int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
 {
-    const int fpl = std::max(1, e->frames_per_launch);
-    const int nb = (fs.nframes + fpl - 1) / fpl;
+    int fpl = std::max(1, e->frames_per_launch);
+    int nb = (fs.nframes + fpl - 1) / fpl;
     JobShape j = job_shape(e, fields, fs.z_flat, fs.n);
     int rc;
     if ((rc = ensure_events(e, (size_t)nb * 3))) return rc;
     if ((rc = e->d_maxcount.ensure(2)) || (rc = e->d_stats.ensure(2))) return rc;
     if (j.pairs && e->cap < 1) e->cap = j.D == 3 ? 32 : 16;
     int mx = 0;
-    for (int attempt = 0; attempt < 3; ++attempt) {
+    for (int attempt = 0; attempt < 4; ++attempt) {
         Work w{};
         if ((rc = work_from_engine(e, j, std::min(fpl, fs.nframes), fs.n, false, w))) return rc;
         if ((rc = fill_async(w.maxcount, 0, 2, e->stream)) || (rc = fill_async(w.stats, 0ull, 2, e->stream))) return rc;
@@ -786,12 +807,30 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
         HIPCHK(hipMemcpyAsync(flags, e->d_maxcount.p, sizeof(flags), hipMemcpyDeviceToHost, e->stream));
         HIPCHK(hipStreamSynchronize(e->stream));
         mx = flags[0];
-        if (flags[1] & 1)
-            return fail(PGT_ERR_RANGE, "an adaptive bond integral needed more than " + std::to_string(kQagMax) +
-                                           " intervals (the GPU rule's workspace)");
         const bool grow = j.pairs && mx > e->cap;
         if (grow && mx > 65535) return fail(PGT_ERR_RANGE, "a particle has more than 65535 interacting neighbours");
         if (grow && e->held) return fail_held();  // the graph's launches keep the old capacity
+        if (flags[1] & 1) {
+            // an adaptive integral needed more than the kQagMax intervals of
+            // the private tables while the caller's limit allows more: run
+            // the job again with tables of the limit in global memory, as
+            // many frames per launch as kQagWorkspaceBytes holds
+            if (j.qag_global) return fail(PGT_ERR_STATE, "adaptive workspace overflow with the caller's limit");
+            const size_t per_frame = (size_t)((e->G + 127) / 128) * 128 * 4 * (size_t)e->prm.adapt_limit;
+            const size_t fit = kQagWorkspaceBytes / sizeof(double) / per_frame;
+            if (fit < 1)
+                return fail(PGT_ERR_RANGE, "an adaptive bond integral needs more than " + std::to_string(kQagMax) +
+                                               " intervals, and the tables of the caller's limit for one frame's "
+                                               "grid exceed the " + std::to_string(kQagWorkspaceBytes >> 20) +
+                                               " MiB workspace");
+            fpl = (int)std::min<size_t>((size_t)fpl, fit);
+            nb = (fs.nframes + fpl - 1) / fpl;
+            if ((rc = ensure_events(e, (size_t)nb * 3))) return rc;
+            if ((rc = e->d_qag.ensure(per_frame * (size_t)std::min(fpl, fs.nframes)))) return rc;
+            j.qag_global = true;
+            if (grow) e->cap = ((mx + 7) / 8) * 8;
+            continue;
+        }
         if (flags[1] & 6) {
             // a tile item had more bonds than its one-byte count holds:
             // run the job again with the lane gather
@@ -801,7 +840,7 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
             break;
         }
         if (grow) e->cap = ((mx + 7) / 8) * 8;
-        if (attempt == 2) return fail(PGT_ERR_HIP, "bond list capacity did not settle");
+        if (attempt == 3) return fail(PGT_ERR_HIP, "bond list capacity did not settle");
     }
     unsigned long long h[2] = {0, 0};
     HIPCHK(hipMemcpy(h, e->d_stats.p, sizeof(h), hipMemcpyDeviceToHost));
@@ -1179,7 +1218,8 @@ int pgt_ik_finish(pgt_ik_engine *e)
     e->stats.bond_capacity = e->cap;
     if (flags[1] & 1)
         return fail(PGT_ERR_RANGE, "an adaptive bond integral needed more than " + std::to_string(kQagMax) +
-                                       " intervals (the GPU rule's workspace)");
+                                       " intervals: the enqueued results are incomplete; pgt_ik_compute_frames "
+                                       "keeps tables of the caller's limit");
     if (flags[1] & 6)
         return fail(PGT_ERR_RANGE, "a particle had more than 255 bonds, more than the tile gather holds: "
                                    "the enqueued results are incomplete; select the lane gather "
@@ -1397,7 +1437,8 @@ int pgt_ik_compute_frames_noalloc(pgt_ik_engine *e, int nframes, int64_t n, cons
         mx = flags[0];
         if (flags[1] & 1)
             return fail(PGT_ERR_RANGE, "an adaptive bond integral needed more than " + std::to_string(kQagMax) +
-                                           " intervals (the GPU rule's workspace)");
+                                           " intervals, which this entry point has no memory for; "
+                                           "pgt_ik_compute_frames keeps tables of the caller's limit");
         const bool grow = j.pairs && mx > cap;
         if (grow && mx > 65535) return fail(PGT_ERR_RANGE, "a particle has more than 65535 interacting neighbours");
         if (flags[1] & 6) {
@@ -1488,10 +1529,17 @@ int pgt_eval_cgfunc(int cg_kind, double cg_rcut, int quad_mode, int quad_order, 
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(PGT_ERR_HIP, "no HIP device available: kernel functions are evaluated on the GPU");
     double *d = nullptr;
-    const size_t tot = (size_t)n * 11 + packed.size() + 1;  // + the capped flag
+    // interval tables of the caller's limit when it exceeds kQagMax
+    const size_t wsn = (adapt && adapt_limit > kQagMax) ? (size_t)n * 4 * (size_t)adapt_limit : 0;
+    if (wsn * sizeof(double) > kQagWorkspaceBytes)
+        return fail(PGT_ERR_RANGE, "the adaptive interval tables of " + std::to_string(n) + " integrals at limit " +
+                                       std::to_string(adapt_limit) + " exceed the " +
+                                       std::to_string(kQagWorkspaceBytes >> 20) + " MiB workspace");
+    const size_t tot = (size_t)n * 11 + packed.size() + 1 + wsn;  // + the capped flag + the tables
     HIPCHK(hipMalloc(&d, sizeof(double) * tot));
     double *dx = d, *dri = d + 3 * n, *drij = d + 6 * n, *ds = d + 9 * n, *dout = d + 10 * n, *dgl = d + 11 * n;
-    int *dcap = reinterpret_cast<int *>(d + tot - 1);
+    int *dcap = reinterpret_cast<int *>(d + 11 * n + packed.size());
+    double *dws = wsn ? d + 11 * n + packed.size() + 1 : nullptr;
     hipError_t he = hipMemset(dcap, 0, sizeof(int));
     if (he == hipSuccess) he = hipMemcpy(dx, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice);
     if (he == hipSuccess) he = hipMemcpy(dri, ri, sizeof(double) * 3 * n, hipMemcpyHostToDevice);
@@ -1501,7 +1549,8 @@ int pgt_eval_cgfunc(int cg_kind, double cg_rcut, int quad_mode, int quad_order, 
         he = hipMemcpy(dgl, packed.data(), sizeof(double) * packed.size(), hipMemcpyHostToDevice);
     if (he == hipSuccess) {
         hipLaunchKernelGGL(k_eval_cg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, cg_kind, cg_rcut, mode,
-                           (int)n, s ? ds : nullptr, dx, dri, rij ? drij : nullptr, dgl, nq, adapt, Q, dcap, dout);
+                           (int)n, s ? ds : nullptr, dx, dri, rij ? drij : nullptr, dgl, nq, adapt, Q, dcap, dws,
+                           dout);
         he = hipGetLastError();
     }
     if (he == hipSuccess) he = hipDeviceSynchronize();
@@ -1510,9 +1559,7 @@ int pgt_eval_cgfunc(int cg_kind, double cg_rcut, int quad_mode, int quad_order, 
     if (he == hipSuccess) he = hipMemcpy(&capped, dcap, sizeof(int), hipMemcpyDeviceToHost);
     (void)hipFree(d);
     if (he != hipSuccess) return fail(PGT_ERR_HIP, std::string("kernel evaluation: ") + hipGetErrorString(he));
-    if (capped)
-        return fail(PGT_ERR_RANGE, "an adaptive bond integral needed more than " + std::to_string(kQagMax) +
-                                       " intervals (the GPU rule's workspace)");
+    if (capped) return fail(PGT_ERR_STATE, "adaptive interval tables smaller than the caller's limit");
     return PGT_OK;
 }
 

@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ik_bonds.hpp"
 #include "ik_device.hpp"
 #include "ik_geom.hpp"
@@ -16,9 +18,11 @@ namespace pgt {
 // by QAG (qag_dev), the integrand normalised as the reference integrates it
 // (CoarseGrainFunction.h:42-45), so the tolerance tests see the same values.
 // The kernel shape is chosen at run time (cg), the bond list is the FP64
-// layout of BondLayout.  Sums are FP64 in candidate order.  A bond integral
-// that needs more than kQagMax intervals sets *capped.
-template <int D, int OD, bool VIR, bool ELA>
+// layout of BondLayout.  Sums are FP64 in candidate order.  GWS false: the
+// interval table of each integral in private memory (kQagMax intervals), an
+// integral that needs more sets *capped; GWS true: in the global workspace
+// ws, wcap intervals per lane (the caller's limit), lane-interleaved.
+template <int D, int OD, bool VIR, bool ELA, bool GWS>
 __global__ __launch_bounds__(128) void k_gather_adaptive(Geom g, int n, int G, const double *__restrict__ grid,
                                                          const int *__restrict__ perm, int cg, AdaptQ Q,
                                                          double cgnorm, const int *__restrict__ cstart,
@@ -26,7 +30,7 @@ __global__ __launch_bounds__(128) void k_gather_adaptive(Geom g, int n, int G, c
                                                          const double *__restrict__ skin, int cap,
                                                          const int *__restrict__ bcount,
                                                          const double *__restrict__ bonds, GatherOut out,
-                                                         int *__restrict__ capped)
+                                                         int *__restrict__ capped, double *__restrict__ ws, int wcap)
 {
     constexpr int BLK = 128;
     constexpr int PS = (D == 2) ? 2 : 4;
@@ -51,6 +55,14 @@ __global__ __launch_bounds__(128) void k_gather_adaptive(Geom g, int n, int G, c
     unsigned long long nball = 0, ncontrib = 0;
     bool hit_cap = false;
     const int gp = live ? perm[t] : 0;
+    using WS = typename std::conditional<GWS, QagGlobal, QagPrivate>::type;
+    WS wsp;
+    if constexpr (GWS) {
+        const size_t stride = (size_t)gridDim.x * gridDim.y * BLK;
+        wsp.p = ws + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * BLK + lane;
+        wsp.stride = stride;
+        wsp.n = wcap;
+    }
 
     if (live) {
         const double x0 = grid[(size_t)gp * 3], x1 = grid[(size_t)gp * 3 + 1];
@@ -93,7 +105,7 @@ __global__ __launch_bounds__(128) void k_gather_adaptive(Geom g, int n, int G, c
                                 }
                                 return cgnorm * shape_rt(cg, tt);
                             };
-                            const double w = qag_dev(obj, Q, hit_cap) / cgnorm;
+                            const double w = qag_dev(obj, Q, hit_cap, wsp) / cgnorm;
                             if (VIR) {
                                 const double c = w * e[LY::OC];
                                 av[0] = fma(c * b[0], b[0], av[0]);

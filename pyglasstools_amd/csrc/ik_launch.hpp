@@ -58,9 +58,13 @@ void launch_gather_2d32(const LaunchCtx &c, unsigned fields, int cg);  // ik_gat
 void launch_gather_3d2(const LaunchCtx &c, unsigned fields, int cg);   // ik_gather_3d2.hip
 void launch_gather_3d64(const LaunchCtx &c, unsigned fields, int cg);  // ik_gather_3d64.hip
 void launch_gather_3d32(const LaunchCtx &c, unsigned fields, int cg);  // ik_gather_3d32.hip
-// Field gather with the adaptive (QAG) bond integral; capped gets 1 when an
-// integral needs more intervals than the workspace holds.
-void launch_gather_adaptive(const LaunchCtx &c, unsigned fields, int cg, int D, int OD, AdaptQ Q,
-                            int *capped);  // ik_gather_adaptive.hip
+// Field gather with the adaptive (QAG) bond integral.  ws null: each
+// integral's interval table in the lane's private memory (kQagMax
+// intervals), and capped gets 1 when an integral needs more; else the global
+// workspace ws of adaptive_workspace_doubles(c, wcap) doubles holds wcap
+// intervals per lane.
+void launch_gather_adaptive(const LaunchCtx &c, unsigned fields, int cg, int D, int OD, AdaptQ Q, int *capped,
+                            double *ws, int wcap);
+size_t adaptive_workspace_doubles(const LaunchCtx &c, int wcap);  // ik_gather_adaptive.hip
 
 }  // namespace pgt

@@ -623,19 +623,40 @@ def test_eval_adaptive_bond_matches_oracle(gpu, cg):
     assert tight >= 0.98 * len(range(0, n, 3))
 
 
-def test_eval_adaptive_workspace_limit_reported(gpu):
+def test_eval_adaptive_caller_limit_beyond_private_tables(gpu):
     """A zero tolerance never converges on a segment that leaves the ball (the
-    Rect kernel jumps there): with a limit above the GPU rule's 128 intervals
-    the evaluator reports a range error instead of a truncated result; with a
-    limit within them it returns what the rule reached."""
+    Rect kernel jumps there), so QAG runs to the caller's limit: 40 intervals
+    fit the private tables, 1000 take tables of the limit in global memory
+    (Quadrature.h:33-35 sizes GSL's workspace by the limit); both return what
+    the rule reached, equal to the oracle's QAG at that limit."""
     x = np.zeros((4, 3))
     ri = np.array([[1.5, 0.3, 0.0]] * 4)
     rij = np.array([[1.1, 0.2, 0.0]] * 4)
-    with pytest.raises(engine.PGTError):
-        engine.eval_cgfunc("bond", x, ri, rij, cg="rect", adaptive=(1000, 0.0, 0.0))
-    got = engine.eval_cgfunc("bond", x, ri, rij, cg="rect", adaptive=(40, 0.0, 0.0))
-    ref = oracle.qag("rect", 2.0, x[0], ri[0], rij[0], 0.0, 0.0, 40)
-    assert got[0] == pytest.approx(ref, rel=1e-12)
+    for limit in (40, 1000):
+        got = engine.eval_cgfunc("bond", x, ri, rij, cg="rect", adaptive=(limit, 0.0, 0.0))
+        ref = oracle.qag("rect", 2.0, x[0], ri[0], rij[0], 0.0, 0.0, limit)
+        print("limit %d: gpu %.17g oracle %.17g" % (limit, got[0], ref), flush=True)
+        assert got[0] == pytest.approx(ref, rel=1e-12)
+
+
+def test_adaptive_field_beyond_private_tables(gpu):
+    """A field whose bond integrals need more intervals than the private
+    tables hold (Rect kernel, zero tolerance, limit 200): the job runs again
+    with tables of the caller's limit in global memory and matches the
+    oracle's QAG at that limit."""
+    sysd, grid = workload.config_frame(1, 2)
+    grid = grid[::16]
+    d = float(np.max(sysd["diam"]))
+    max_rcut = max(1.25 * d, d)
+    params = oracle.poly12_params()
+    adapt = (200, 0.0, 0.0)
+    e = engine.Engine(dim=2, params=params, cg="rect", max_rcut=max_rcut, adaptive=adapt)
+    e.set_system(sysd, grid)
+    got = e.compute(fields=("virial",))
+    st = e.stats()
+    ref = oracle.ik_compute(sysd, grid, params=params, max_rcut=max_rcut, cg="rect", adaptive=adapt,
+                            fields=("virial",), with_abs=True)
+    assert_parity(got, ref, st, ("virial",), tol=1e-8)
 
 
 @pytest.mark.parametrize("cg", ["octic", "rect"])
