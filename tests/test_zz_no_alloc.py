@@ -3,7 +3,8 @@ work arrays live in a static device scratch array) against
 pgt_ik_compute_frames on the same inputs, through the C ABI.  Two calls in a
 row, the second with fewer frames and fewer particles (the scratch array
 keeps the first call's contents, so anything left uncleared would show), for
-the FP64 virial + kinetic form, the elastic field and the 3D FP32 form; and a
+the FP64 virial + kinetic form, the elastic field and the 3D FP32 form, with
+the lane and the tile gather; and a
 frame too large for the scratch array is refused before any launch."""
 import numpy as np
 import pytest
@@ -23,18 +24,21 @@ def _stack(frames, key, dev):
     return torch.tensor(np.stack([f[key] for f in frames]), dtype=torch.float64, device=dev)
 
 
+@pytest.mark.parametrize("gather", ["lane", "tile"])
 @pytest.mark.parametrize("dim,precision,fields,ng", [
     (2, "fp64", ("virial", "kinetic"), 32),
     (2, "fp64", ("virial", "kinetic", "elastic"), 24),
     (3, "fp32", ("virial",), 12),
 ])
-def test_noalloc_matches_compute_frames(gpu, dim, precision, fields, ng):
+def test_noalloc_matches_compute_frames(gpu, dim, precision, fields, ng, gather):
     import torch
 
     dev = torch.device("cuda:0")
     e = engine.Engine(dim=dim, max_rcut=max(1.25 * 1.62, 1.62), precision=precision)
+    e.set_gather_mode(gather)
     for call, (N, nf, seed0) in enumerate(((3000, 3, 7100), (1500, 2, 7200))):
-        print("noalloc: dim %d %s %s call %d N=%d frames=%d" % (dim, precision, fields, call, N, nf), flush=True)
+        print("noalloc: %s gather, dim %d %s %s call %d N=%d frames=%d" % (gather, dim, precision, fields, call, N,
+                                                                          nf), flush=True)
         frames = _frames(N, dim, nf, seed0)
         L = frames[0]["L"]
         grid = workload.cartesian_grid_2d(L[0], ng) if dim == 2 else workload.cartesian_grid_3d(L[0], ng)
