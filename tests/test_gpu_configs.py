@@ -1,11 +1,12 @@
 """The benchmark workloads at their own sizes, against the FP64 oracle,
 through the C ABI:
 
-  2D N=65,536, 100 frames, 256 x 256 grid, FP64 virial + kinetic: frames 0,
-    50 and 99 on every grid point at the drop-in tolerance, contribution and
-    ball counts of all 100 frames exactly;
-  3D N=131,072, 20 frames, 64^3 grid, FP32 form: frames 0 and 19 on 2,048
-    sampled points, each component within 1e-5 of its largest |T|; the FP64
+  2D N=65,536, 100 frames, 256 x 256 grid, FP64 virial + kinetic, with the
+    tile and the lane gather: frames 0, 50 and 99 on every grid point at the
+    drop-in tolerance, contribution and ball counts of all 100 frames exactly;
+  3D N=131,072, 20 frames, 64^3 grid, FP32 form (tile gather at cell reach
+    2, and the lane gather): frames 0 and 19 on 2,048 sampled points, each
+    component within 1e-5 of its largest |T|; the FP64
     form on the same points at the drop-in tolerance with exact counts; and
     over all 20 frames the FP32 job's ball count equal to the FP64 job's and
     its contribution count within 1e-6 of it (pairs at the cutoff);
@@ -77,14 +78,19 @@ def test_c2_fp64_100_frames(gpu):
     e = engine.Engine(dim=2, max_rcut=max_rcut)
     e.set_box(L, [1, 1, 0])
     e.set_grid(grid)
-    e.set_tuning(threads_per_block=128, frames_per_launch=F, points_per_lane=2, cell_reach=1)
     pos, diam, mass, vel = (_stack(frames, k, dev) for k in ("pos", "diam", "mass", "vel"))
-    ov = torch.full((F, G, 2, 2), np.nan, dtype=torch.float64, device=dev)
-    ok = torch.full_like(ov, np.nan)
-    print("c2: one 100-frame job", flush=True)
-    e.compute_frames(F, N, pos, diam, mass, vel, None, fields=("virial", "kinetic"), d_virial=ov, d_kinetic=ok)
-    torch.cuda.synchronize()
-    st = e.stats()
+    # both field gathers, each as one 100-frame job: the tile gather (the
+    # benchmark's choice) and the lane gather with two points per lane
+    runs = {}
+    for gather, pl in (("tile", 1), ("lane", 2)):
+        e.set_gather_mode(gather)
+        e.set_tuning(threads_per_block=128, frames_per_launch=F, points_per_lane=pl, cell_reach=1)
+        ov = torch.full((F, G, 2, 2), np.nan, dtype=torch.float64, device=dev)
+        ok = torch.full_like(ov, np.nan)
+        print("c2: one 100-frame job, %s gather" % gather, flush=True)
+        e.compute_frames(F, N, pos, diam, mass, vel, None, fields=("virial", "kinetic"), d_virial=ov, d_kinetic=ok)
+        torch.cuda.synchronize()
+        runs[gather] = (ov, ok, e.stats())
     n_nonzero = n_ball = 0
     for f in range(F):
         ref = oracle.ik_compute(frames[f], grid, fields=("virial", "kinetic"), max_rcut=max_rcut,
@@ -92,15 +98,17 @@ def test_c2_fp64_100_frames(gpu):
         n_nonzero += ref["n_nonzero"]
         n_ball += ref["n_ball"]
         if f in (0, 50, 99):
-            gv, gk = ov[f].cpu().numpy(), ok[f].cpu().numpy()
-            assert_fp64_parity(gv, ref["virial"], ref["virial_abs"], "c2 frame %d virial" % f)
-            ek = np.abs(gk - ref["kinetic"]).max() / np.abs(ref["kinetic"]).max()
-            print("c2 frame %d: kinetic %.3e" % (f, ek), flush=True)
-            assert ek <= TOL
-    print("c2: contributions %d (oracle %d), balls %d (oracle %d)" % (st["n_contrib"], n_nonzero, st["n_ball"],
-                                                                        n_ball), flush=True)
-    assert st["n_contrib"] == n_nonzero
-    assert st["n_ball"] == n_ball
+            for gather, (ov, ok, _) in runs.items():
+                gv, gk = ov[f].cpu().numpy(), ok[f].cpu().numpy()
+                assert_fp64_parity(gv, ref["virial"], ref["virial_abs"], "c2 %s frame %d virial" % (gather, f))
+                ek = np.abs(gk - ref["kinetic"]).max() / np.abs(ref["kinetic"]).max()
+                print("c2 %s frame %d: kinetic %.3e" % (gather, f, ek), flush=True)
+                assert ek <= TOL
+    for gather, (_, _, st) in runs.items():
+        print("c2 %s: contributions %d (oracle %d), balls %d (oracle %d)" % (gather, st["n_contrib"], n_nonzero,
+                                                                            st["n_ball"], n_ball), flush=True)
+        assert st["n_contrib"] == n_nonzero
+        assert st["n_ball"] == n_ball
 
 
 def test_c3_fp32_form_20_frames(gpu):
@@ -118,32 +126,37 @@ def test_c3_fp32_form_20_frames(gpu):
     pos, diam = _stack(frames, "pos", dev), _stack(frames, "diam", dev)
     out = {}
     counts = {}
-    for prec in ("fp32", "fp64"):
-        print("c3: %s 20-frame job" % prec, flush=True)
+    # the FP32 form with the tile gather at cell reach 2 (the benchmark's
+    # autotuned shape) and with the lane gather; the FP64 form beside them
+    for name, prec, gather, reach in (("fp32", "fp32", "tile", 2), ("fp32_lane", "fp32", "lane", 1),
+                                      ("fp64", "fp64", "lane", 1)):
+        print("c3: %s 20-frame job, %s gather, reach %d" % (prec, gather, reach), flush=True)
         e = engine.Engine(dim=3, max_rcut=max_rcut, precision=prec)
         e.set_box(L, [1, 1, 1])
         e.set_grid(grid)
-        e.set_tuning(frames_per_launch=F)
+        e.set_gather_mode(gather)
+        e.set_tuning(frames_per_launch=F, cell_reach=reach)
         ov = torch.full((F, G, 3, 3), np.nan, dtype=torch.float64, device=dev)
         e.compute_frames(F, N, pos, diam, None, None, None, fields=("virial",), d_virial=ov)
         torch.cuda.synchronize()
-        out[prec] = ov
-        counts[prec] = (e.stats()["n_contrib"], e.stats()["n_ball"])
+        out[name] = ov
+        counts[name] = (e.stats()["n_contrib"], e.stats()["n_ball"])
         e.close()
-    print("c3: counts fp32 %s fp64 %s" % (counts["fp32"], counts["fp64"]), flush=True)
+    print("c3: counts %s" % counts, flush=True)
     # the same particles in every ball; the FP32 form decides the pair cutoff
     # in FP32, so pairs within FP32 rounding of it (where the force is ~0) may
     # count on one side only
-    assert counts["fp32"][1] == counts["fp64"][1]
-    assert abs(counts["fp32"][0] - counts["fp64"][0]) <= 1e-6 * counts["fp64"][0]
+    for name in ("fp32", "fp32_lane"):
+        assert counts[name][1] == counts["fp64"][1]
+        assert abs(counts[name][0] - counts["fp64"][0]) <= 1e-6 * counts["fp64"][0]
     idx = np.sort(np.random.default_rng(7).choice(G, size=2048, replace=False))
     for f in (0, F - 1):
         ref = oracle.ik_compute(frames[f], grid[idx], fields=("virial",), max_rcut=max_rcut, with_abs=True)
-        v32 = out["fp32"][f].cpu().numpy()[idx]
         v64 = out["fp64"][f].cpu().numpy()[idx]
-        e32 = _fp32_rel(v32, ref["virial"])
-        print("c3 frame %d: fp32 worst component err %.3e" % (f, e32), flush=True)
-        assert e32 <= TOL_FP32
+        for name in ("fp32", "fp32_lane"):
+            e32 = _fp32_rel(out[name][f].cpu().numpy()[idx], ref["virial"])
+            print("c3 frame %d: %s worst component err %.3e" % (f, name, e32), flush=True)
+            assert e32 <= TOL_FP32
         assert_fp64_parity(v64, ref["virial"], ref["virial_abs"], "c3 frame %d fp64 form" % f)
         # exact counts on the sampled points: one FP64 frame with the sample as its grid
         e = engine.Engine(dim=3, max_rcut=max_rcut)
