@@ -64,7 +64,7 @@ void launch_tile_t(const LaunchCtx &c)
     const dim3 grid((unsigned)((c.G + CP::TP - 1) / CP::TP), (unsigned)c.nf);
     c.pl_launched = 1;
     hipLaunchKernelGGL((k_gather_tile<D, OD, 0, 5, VIR, ELA, KIN, ER>), grid, dim3(CP::BLK), 0, c.stream, c.g, c.n, c.G,
-                       c.grid, c.perm, c.R, c.gld, c.cgnorm, c.cstart, c.spos, c.skin, c.cap, c.bcount,
+                       c.grid, c.perm, c.R, c.gld, c.cgnorm, c.cstart, c.spos, c.skin, c.cap, c.bcount, c.rec,
                        (const ER *)c.bonds, c.out, c.flags);
 }
 

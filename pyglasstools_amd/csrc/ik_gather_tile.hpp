@@ -8,8 +8,10 @@
 //
 // 1. Scan: each lane walks the cells around its point, as the lane gather
 //    does, in steps of NB candidates that every lane takes together, and
-//    tests them exactly (FP64 minimum image, |a|^2 < r_cg^2).  The items
-//    found go to a list shared by the wavefront (ballot, prefix count).
+//    tests them exactly (FP64 minimum image, |a|^2 < r_cg^2; in 3D with
+//    bonds an FP32 test on the bond kernel's records first, with the FP64
+//    test for the band where FP32 rounding could decide).  The items found
+//    go to a list shared by the wavefront (ballot, prefix count).
 // 2. When the list could overflow, and at the end: the list is sorted by
 //    the particle's bond count, largest first (stable counting sort), and
 //    lane l takes items l, l + 64, ...: the 64 items of a step have nearly
@@ -62,13 +64,11 @@ struct TileWaves {
 };
 
 template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, bool KIN, typename ER>
-__global__ __launch_bounds__(64, (TileWaves<D, ELA>::value)) void k_gather_tile(Geom g, int n, int G, const double *__restrict__ grid,
-                                                    const int *__restrict__ perm, GLRule R,
-                                                    const double *__restrict__ gld, double cgnorm,
-                                                    const int *__restrict__ cstart, const double *__restrict__ spos,
-                                                    const double *__restrict__ skin, int cap,
-                                                    const int *__restrict__ bcount, const ER *__restrict__ bonds,
-                                                    GatherOut out, int *__restrict__ overflow)
+__global__ __launch_bounds__(64, (TileWaves<D, ELA>::value)) void k_gather_tile(
+    Geom g, int n, int G, const double *__restrict__ grid, const int *__restrict__ perm, GLRule R,
+    const double *__restrict__ gld, double cgnorm, const int *__restrict__ cstart, const double *__restrict__ spos,
+    const double *__restrict__ skin, int cap, const int *__restrict__ bcount, const float4 *__restrict__ rec,
+    const ER *__restrict__ bonds, GatherOut out, int *__restrict__ overflow)
 {
     using CP = TileCaps;
     using LY = BondLayout<D, OD, ELA, ER>;
@@ -95,7 +95,6 @@ __global__ __launch_bounds__(64, (TileWaves<D, ELA>::value)) void k_gather_tile(
 
     const int f = blockIdx.y;
     const int l = threadIdx.x;
-    const unsigned long long below = l ? (~0ull >> (64 - l)) : 0ull;
     const int t0 = xcd_tile(blockIdx.x, gridDim.x) * 64;
     const int t = t0 + l;
     const bool live = t < G;
@@ -104,6 +103,10 @@ __global__ __launch_bounds__(64, (TileWaves<D, ELA>::value)) void k_gather_tile(
     const int *bc = bcount + (size_t)f * n;
     const ER *bd = bonds + (size_t)f * n * (size_t)cap * ES;
     const double *sk = KIN ? skin + (size_t)f * n * KN : nullptr;
+    // 3D with bonds: the scan reads the bond kernel's 16-byte records (FP32
+    // position, bond count) and tests in FP32 first, as the lane gather does
+    constexpr bool RECS = PAIRS && D == 3;
+    const float4 *rf = RECS ? rec + (size_t)f * n : nullptr;
 
     const int gp = perm[live ? t : t0];
     double xg[3];
@@ -332,18 +335,37 @@ __global__ __launch_bounds__(64, (TileWaves<D, ELA>::value)) void k_gather_tile(
     // empty row reads particle 0, and every lane has one when any reads)
     struct Cand {
         int p[NB], nb[NB];
-        double q[NB][3];
+        double q[RECS ? 1 : NB][3];
+        float r[RECS ? NB : 1][3];
     };
     auto load = [&](int c0, int b0, int len0, int b1, int len, Cand &C) {
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             const int c = min(c0 + u, len - 1);
             C.p[u] = len == 0 ? 0 : (c < len0 ? b0 + c : b1 + (c - len0));
-            C.q[u][0] = sp[(size_t)C.p[u] * PS];
-            C.q[u][1] = sp[(size_t)C.p[u] * PS + 1];
-            C.q[u][2] = (D == 3) ? sp[(size_t)C.p[u] * PS + 2] : 0.0;
-            C.nb[u] = PAIRS ? bc[C.p[u]] : 0;
+            if constexpr (RECS) {
+                const float4 r4 = rf[C.p[u]];
+                C.r[u][0] = r4.x;
+                C.r[u][1] = r4.y;
+                C.r[u][2] = r4.z;
+                C.nb[u] = __float_as_int(r4.w);
+            } else {
+                C.q[u][0] = sp[(size_t)C.p[u] * PS];
+                C.q[u][1] = sp[(size_t)C.p[u] * PS + 1];
+                C.q[u][2] = (D == 3) ? sp[(size_t)C.p[u] * PS + 2] : 0.0;
+                C.nb[u] = PAIRS ? bc[C.p[u]] : 0;
+            }
         }
+    };
+    float xg32[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) xg32[a] = (float)xg[a];
+    // the exact test: FP64 minimum image and |a|^2 against r_cg^2
+    auto exact_in = [&](const double *q) {
+        const double a0 = minimg(q[0] - xg[0], g.L[0], g.pinvL[0]);
+        const double a1 = minimg(q[1] - xg[1], g.L[1], g.pinvL[1]);
+        const double a2 = (D == 3) ? minimg(q[2] - xg[2], g.L[2], g.pinvL[2]) : 0.0;
+        return a0 * a0 + a1 * a1 + a2 * a2 < g.rc2;
     };
     bool over = false;  // an item's bond count exceeds MAXNB
     // the scan visits rows r = 0 .. nrows - 1, each in steps of NB candidates
@@ -351,11 +373,27 @@ __global__ __launch_bounds__(64, (TileWaves<D, ELA>::value)) void k_gather_tile(
     auto test_append = [&](const Cand &C, int c0, int len, int &ni) {
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
-            const double a0 = minimg(C.q[u][0] - xg[0], g.L[0], g.pinvL[0]);
-            const double a1 = minimg(C.q[u][1] - xg[1], g.L[1], g.pinvL[1]);
-            const double a2 = (D == 3) ? minimg(C.q[u][2] - xg[2], g.L[2], g.pinvL[2]) : 0.0;
-            const double asq = a0 * a0 + a1 * a1 + a2 * a2;
-            const bool in = c0 + u < len && asq < g.rc2;
+            bool in;
+            if constexpr (RECS) {
+                // FP32 pre-test against bounds of the FP64 rounding; a
+                // candidate in the band between them (or any, when the
+                // geometry gives no bound) is decided by the exact test
+                float d0 = C.r[u][0] - xg32[0], d1 = C.r[u][1] - xg32[1], d2 = C.r[u][2] - xg32[2];
+                d0 = fmaf(-g.L32[0], rintf(d0 * g.pinvL32[0]), d0);
+                d1 = fmaf(-g.L32[1], rintf(d1 * g.pinvL32[1]), d1);
+                d2 = fmaf(-g.L32[2], rintf(d2 * g.pinvL32[2]), d2);
+                const float a2f = d0 * d0 + d1 * d1 + d2 * d2;
+                in = a2f < g.in32;
+                if (!in && !(a2f > g.out32) && c0 + u < len) {
+                    double q[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) q[a] = sp[(size_t)C.p[u] * PS + a];
+                    in = exact_in(q);
+                }
+                in = in && c0 + u < len;
+            } else {
+                in = c0 + u < len && exact_in(C.q[u]);
+            }
             const unsigned long long bm = __builtin_amdgcn_ballot_w64(in);
             if (in) {
                 const int k = ni + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32),

@@ -20,7 +20,7 @@ TOL = 1e-10
 
 
 def run_both(sysd, grid, fields=("virial",), dim=None, order=5, cg="octic", cg_rcut=2.0, pot="poly12", params=None,
-             scaled_rcut=1.25, max_rcut=None, block=None, precision="fp64", pl=None, reach=None):
+             scaled_rcut=1.25, max_rcut=None, block=None, precision="fp64", pl=None, reach=None, gather=None):
     dim = dim or sysd["dim"]
     params = params or oracle.poly12_params(rcut=scaled_rcut)
     if max_rcut is None:
@@ -34,6 +34,8 @@ def run_both(sysd, grid, fields=("virial",), dim=None, order=5, cg="octic", cg_r
         e.set_tuning(points_per_lane=pl)
     if reach:
         e.set_tuning(cell_reach=reach)
+    if gather:
+        e.set_gather_mode(gather)
     e.set_system(sysd, grid)
     got = e.compute(fields=fields)
     st = e.stats()
@@ -198,15 +200,16 @@ def test_3d_small(gpu):
     assert_parity(got, ref, st, ("virial", "kinetic"))
 
 
+@pytest.mark.parametrize("gather", ["lane", "tile"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_3d_candidates_on_the_ball_surface(gpu, precision):
-    """The 3D candidate scan tests distances in FP32 first and leaves those
-    near r_cg to the FP64 test.  Grid points placed at r_cg (1 +- 1e-13 ...
-    1e-5) from particles, in random directions, put many candidates in that
-    band; which particles are inside a ball must still be exactly the
-    oracle's, as must the fields.  (Offsets stay well above the last bit of
-    |a|^2, where the FP64 sum itself may round differently with or without
-    fused multiply-adds.)"""
+def test_3d_candidates_on_the_ball_surface(gpu, precision, gather):
+    """The 3D candidate scans (lane and tile gather) test distances in FP32
+    first and leave those near r_cg to the FP64 test.  Grid points placed at
+    r_cg (1 +- 1e-13 ... 1e-5) from particles, in random directions, put many
+    candidates in that band; which particles are inside a ball must still be
+    exactly the oracle's, as must the fields.  (Offsets stay well above the
+    last bit of |a|^2, where the FP64 sum itself may round differently with
+    or without fused multiply-adds.)"""
     sysd = workload.make_system(3000, 3, seed=71)
     rng = np.random.default_rng(8)
     idx = rng.choice(len(sysd["diam"]), 160, replace=False)
@@ -217,7 +220,7 @@ def test_3d_candidates_on_the_ball_surface(gpu, precision):
     # back into the box, as the particles are
     L = sysd["L"]
     grid = grid - L * np.floor((grid + L / 2) / L)
-    got, ref, st = run_both(sysd, grid, fields=("virial", "kinetic"), precision=precision)
+    got, ref, st = run_both(sysd, grid, fields=("virial", "kinetic"), precision=precision, gather=gather)
     assert st["n_ball"] == ref["n_ball"]
     if precision == "fp64":
         assert_parity(got, ref, st, ("virial", "kinetic"))
