@@ -1,5 +1,6 @@
 // ik_gather_launch.hpp -- template dispatch from run-time settings (fields,
-// kernel, workgroup size, points per lane) to the k_gather instantiations.
+// kernel, gather kind, workgroup size, points per lane) to the k_gather and
+// k_gather_tile instantiations.
 // Included by the ik_gather_<form>.hip translation units only.
 #pragma once
 #include <hip/hip_runtime.h>
