@@ -103,8 +103,9 @@ def test_tile_elastic_and_fp32_2d(gpu):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_tile_3d_several_chunks(gpu, precision):
-    """3D windows hold more particles than one LDS chunk (384), so every tile
-    stages its window in pieces."""
+    """A 3D tile lists about 2,100 (point, particle) items, more than its
+    768-item list holds, so every tile sorts and integrates its list in
+    several pieces."""
     sysd = workload.make_system(6000, 3, 8400)
     grid = workload.cartesian_grid_3d(sysd["L"][0], 16)
     mr = max(1.25 * float(sysd["diam"].max()), float(sysd["diam"].max()))
@@ -242,3 +243,69 @@ def test_tile_more_than_255_bonds_runs_the_lane_gather(gpu):
     ref = oracle.ik_compute(sysd, grid, params=params, scaled_rcut=rc, fields=("virial",), max_rcut=mr,
                             with_abs=True)
     _check(got, ref, ("virial",), st)
+
+
+def test_tile_irregular_grid_dilute_system_wide_ball(gpu):
+    """Edge cases of the tile gather at once: a grid of 37 x 41 points (tiles
+    not filled to 64), shifted so that a third of it lies outside the box
+    (minimum image and wrapped cells), particles removed from half of the box
+    (empty cells, tiles with no items), and r_cg = 3.5, which puts about
+    40 particles in a ball and makes every busy tile flush its list several
+    times."""
+    sysd = workload.make_system(3000, 2, 8600)
+    keep = sysd["pos"][:, 0] < 0.0
+    sysd = dict(sysd, pos=sysd["pos"][keep], diam=sysd["diam"][keep], mass=sysd["mass"][keep],
+                vel=sysd["vel"][keep])
+    L = sysd["L"]
+    gx = np.linspace(-L[0] / 2, L[0] / 2, 37) + L[0] / 3
+    gy = np.linspace(-L[1] / 2, L[1] / 2, 41)
+    grid = np.zeros((37 * 41, 3))
+    grid[:, 0] = np.repeat(gx, 41)
+    grid[:, 1] = np.tile(gy, 37)
+    mr = max(1.25 * 1.62, 1.62)
+    print("tile 2d dilute N=%d, 37x41 shifted grid, r_cg 3.5" % len(sysd["diam"]), flush=True)
+    e = engine.Engine(dim=2, max_rcut=mr, cg_rcut=3.5)
+    e.set_gather_mode("tile")
+    e.set_system(sysd, grid)
+    got = e.compute(fields=("virial", "kinetic"))
+    ref = oracle.ik_compute(sysd, grid, fields=("virial", "kinetic"), max_rcut=mr, cg_rcut=3.5, with_abs=True)
+    assert (np.abs(ref["virial"]).reshape(len(grid), -1).max(axis=1) == 0).any()  # some empty balls
+    _check(got, ref, ("virial", "kinetic"), e.stats())
+
+
+def test_tile_3d_elastic_and_kinetic(gpu):
+    """3D virial, kinetic and elastic fields.  With the elastic field a bond
+    is listed when either the force at the separation or at the displaced
+    separation is nonzero, so the contribution count is compared with the
+    lane gather's (the oracle counts nonzero-force triples)."""
+    sysd = workload.make_system(2000, 3, 8700)
+    rng = np.random.default_rng(6)
+    sysd["disp"] = rng.normal(0.0, 0.03, (2000, 3))
+    grid = workload.cartesian_grid_3d(sysd["L"][0], 9)
+    mr = max(1.25 * float(sysd["diam"].max()), float(sysd["diam"].max()))
+    print("tile 3d elastic + kinetic N=2000 grid 9^3", flush=True)
+    ref = oracle.ik_compute(sysd, grid, fields=("virial", "kinetic", "elastic"), max_rcut=mr, with_abs=True)
+    stats = {}
+    for mode in ("tile", "lane"):
+        e = _engine(3, mr, mode=mode)
+        e.set_system(sysd, grid)
+        got = e.compute(fields=("virial", "kinetic", "elastic"))
+        _check(got, ref, ("virial", "kinetic", "elastic"))
+        stats[mode] = e.stats()
+    assert stats["tile"]["n_ball"] == stats["lane"]["n_ball"] == ref["n_ball"]
+    assert stats["tile"]["n_contrib"] == stats["lane"]["n_contrib"]
+
+
+def test_tile_tiny_system_single_cell_axes(gpu):
+    """Twenty particles in a box a little over twice the search radius: one
+    cell along each axis, so every window is the whole box."""
+    sysd = workload.make_system(20, 2, 8800)
+    grid = workload.cartesian_grid_2d(sysd["L"][0], 7)
+    mr = max(1.25 * float(sysd["diam"].max()), float(sysd["diam"].max()))
+    assert 2 * mr < sysd["L"][0]
+    print("tile 2d N=20 L=%.2f" % sysd["L"][0], flush=True)
+    e = _engine(2, mr)
+    e.set_system(sysd, grid)
+    got = e.compute(fields=("virial", "kinetic"))
+    ref = oracle.ik_compute(sysd, grid, fields=("virial", "kinetic"), max_rcut=mr, with_abs=True)
+    _check(got, ref, ("virial", "kinetic"), e.stats())
