@@ -518,7 +518,8 @@ JobShape job_shape(const pgt_ik_engine *e, unsigned fields, bool z_flat, int64_t
         j.pl = 1;
         j.gwaves = (e->G + kTilePoints - 1) / kTilePoints * (kTileThreads / 64);
     } else {
-        const int64_t lanes = (e->G + j.pl - 1) / j.pl;
+        // the adaptive gather deals each point's integrals to several lanes
+        const int64_t lanes = j.adaptive ? e->G * adaptive_lanes(e->G) : (e->G + j.pl - 1) / j.pl;
         j.gwaves = (lanes + j.block - 1) / j.block * (j.block / 64);
     }
     return j;
@@ -816,7 +817,8 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
             // the job again with tables of the limit in global memory, as
             // many frames per launch as kQagWorkspaceBytes holds
             if (j.qag_global) return fail(PGT_ERR_STATE, "adaptive workspace overflow with the caller's limit");
-            const size_t per_frame = (size_t)((e->G + 127) / 128) * 128 * 4 * (size_t)e->prm.adapt_limit;
+            const size_t per_frame =
+                (((size_t)e->G * adaptive_lanes(e->G) + 127) / 128) * 128 * 4 * (size_t)e->prm.adapt_limit;
             const size_t fit = kQagWorkspaceBytes / sizeof(double) / per_frame;
             if (fit < 1)
                 return fail(PGT_ERR_RANGE, "an adaptive bond integral needs more than " + std::to_string(kQagMax) +

@@ -10,21 +10,22 @@ template <int D, int OD, bool GWS>
 static void launch_adaptive_w(const LaunchCtx &c, unsigned fields, int cg, AdaptQ Q, int *capped, double *ws,
                               int wcap)
 {
-    const dim3 grid((unsigned)((c.G + 127) / 128), (unsigned)c.nf);
+    const int lpp = adaptive_lanes(c.G);
+    const dim3 grid((unsigned)(((size_t)c.G * lpp + 127) / 128), (unsigned)c.nf);
     const bool V = fields & PGT_FIELD_VIRIAL, E = fields & PGT_FIELD_ELASTIC;
     const double *bonds = (const double *)c.bonds;
     if (V && E)
         hipLaunchKernelGGL((k_gather_adaptive<D, OD, true, true, GWS>), grid, dim3(128), 0, c.stream, c.g, c.n, c.G,
                            c.grid, c.perm, cg, Q, c.cgnorm, c.cstart, c.spos, c.skin, c.cap, c.bcount, bonds, c.out,
-                           capped, ws, wcap);
+                           capped, ws, wcap, lpp);
     else if (E)
         hipLaunchKernelGGL((k_gather_adaptive<D, OD, false, true, GWS>), grid, dim3(128), 0, c.stream, c.g, c.n, c.G,
                            c.grid, c.perm, cg, Q, c.cgnorm, c.cstart, c.spos, c.skin, c.cap, c.bcount, bonds, c.out,
-                           capped, ws, wcap);
+                           capped, ws, wcap, lpp);
     else
         hipLaunchKernelGGL((k_gather_adaptive<D, OD, true, false, GWS>), grid, dim3(128), 0, c.stream, c.g, c.n, c.G,
                            c.grid, c.perm, cg, Q, c.cgnorm, c.cstart, c.spos, c.skin, c.cap, c.bcount, bonds, c.out,
-                           capped, ws, wcap);
+                           capped, ws, wcap, lpp);
 }
 
 template <int D, int OD>
@@ -37,7 +38,7 @@ static void launch_adaptive_d(const LaunchCtx &c, unsigned fields, int cg, Adapt
 
 size_t adaptive_workspace_doubles(const LaunchCtx &c, int wcap)
 {
-    return (size_t)((c.G + 127) / 128) * 128 * (size_t)c.nf * 4 * (size_t)wcap;
+    return (((size_t)c.G * adaptive_lanes(c.G) + 127) / 128) * 128 * (size_t)c.nf * 4 * (size_t)wcap;
 }
 
 void launch_gather_adaptive(const LaunchCtx &c, unsigned fields, int cg, int D, int OD, AdaptQ Q, int *capped,

@@ -8,17 +8,22 @@
 #include "ik_bonds.hpp"
 #include "ik_device.hpp"
 #include "ik_geom.hpp"
+#include "ik_launch.hpp"
 
 namespace pgt {
 
-// One lane per grid point (Morton order, as k_gather).  The lane walks the
-// cells around its point; each particle inside the ball adds its kinetic
-// term, and each of its bonds the pair term with the bond integral
+// lpp lanes per grid point (Morton order, as k_gather), so that even
+// a grid of a thousand points fills the GPU: every lane of a point walks the
+// cells around it, and the point's (particle, bond) integrals are dealt to
+// its lanes round-robin in that walk's order.  Each particle inside the ball
+// adds its kinetic term, and each of its bonds the pair term with the bond
+// integral
 //   int_0^1 Delta(|x - (r_i + s r_ij)|^2) ds
 // by QAG (qag_dev), the integrand normalised as the reference integrates it
 // (CoarseGrainFunction.h:42-45), so the tolerance tests see the same values.
 // The kernel shape is chosen at run time (cg), the bond list is the FP64
-// layout of BondLayout.  Sums are FP64 in candidate order.  GWS false: the
+// layout of BondLayout.  Each lane sums its share in FP64 in walk order, and
+// the point's lanes are added by a fixed butterfly.  GWS false: the
 // interval table of each integral in private memory (kQagMax intervals), an
 // integral that needs more sets *capped; GWS true: in the global workspace
 // ws, wcap intervals per lane (the caller's limit), lane-interleaved.
@@ -30,7 +35,8 @@ __global__ __launch_bounds__(128) void k_gather_adaptive(Geom g, int n, int G, c
                                                          const double *__restrict__ skin, int cap,
                                                          const int *__restrict__ bcount,
                                                          const double *__restrict__ bonds, GatherOut out,
-                                                         int *__restrict__ capped, double *__restrict__ ws, int wcap)
+                                                         int *__restrict__ capped, double *__restrict__ ws, int wcap,
+                                                         int lpp)
 {
     constexpr int BLK = 128;
     constexpr int PS = (D == 2) ? 2 : 4;
@@ -38,9 +44,11 @@ __global__ __launch_bounds__(128) void k_gather_adaptive(Geom g, int n, int G, c
     constexpr bool PAIRS = VIR || ELA;
     using LY = BondLayout<D, OD, ELA, double>;
     constexpr int ES = LY::ES;
+    const int LPP = lpp;  // a power of two from 1 to 32 (adaptive_lanes)
     const int f = blockIdx.y;
     const int lane = threadIdx.x;
-    const int t = xcd_tile(blockIdx.x, gridDim.x) * BLK + lane;
+    const int sub = lane & (LPP - 1);  // this lane's share of the point's integrals
+    const int t = (xcd_tile(blockIdx.x, gridDim.x) * BLK + lane) / LPP;
     const bool live = t < G;
     const bool KIN = skin != nullptr;
     const double *sp = spos + (size_t)f * n * PS;
@@ -64,6 +72,7 @@ __global__ __launch_bounds__(128) void k_gather_adaptive(Geom g, int n, int G, c
         wsp.n = wcap;
     }
 
+    int pcount = 0, bcount_walk = 0;  // particles and bonds met in the walk, for dealing them to lanes
     if (live) {
         const double x0 = grid[(size_t)gp * 3], x1 = grid[(size_t)gp * 3 + 1];
         const double x2 = (D == 3) ? grid[(size_t)gp * 3 + 2] : 0.0;
@@ -81,19 +90,20 @@ __global__ __launch_bounds__(128) void k_gather_adaptive(Geom g, int n, int G, c
                     a[2] = (D == 3) ? minimg(sp[(size_t)p * PS + 2] - x2, g.L[2], g.pinvL[2]) : 0.0;
                     const double asq = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
                     if (!(asq < g.rc2)) continue;
-                    ++nball;
-                    if (KIN) {
+                    if (sub == 0) ++nball;
+                    if (KIN && (pcount++ & (LPP - 1)) == sub) {
                         const double w = shape_rt(cg, asq * g.inv_rc2);
 #pragma unroll
                         for (int k = 0; k < KN; ++k) ak[k] = fma(w, sk[(size_t)p * KN + k], ak[k]);
                     }
                     if constexpr (PAIRS) {
                         const int nb = min(bc[p], cap);
-                        ncontrib += nb;
                         double as[3];  // r_i - x in units of the kernel width
 #pragma unroll
                         for (int k = 0; k < 3; ++k) as[k] = a[k] * g.inv_rc;
                         for (int j = 0; j < nb; ++j) {
+                            if ((bcount_walk++ & (LPP - 1)) != sub) continue;
+                            ++ncontrib;
                             const double *e = bd + ((size_t)p * cap + j) * ES;
                             double b[3] = {e[0], e[1], D == 3 ? e[2] : 0.0};
                             auto obj = [&](double sv) {
@@ -126,6 +136,17 @@ __global__ __launch_bounds__(128) void k_gather_adaptive(Geom g, int n, int G, c
                     }
                 }
         }
+    }
+    // the point's lanes: a fixed butterfly over the LPP lanes of the group
+    // (every lane of a group is live or none is)
+    for (int m = 1; m < LPP; m <<= 1)
+#pragma unroll
+        for (int k = 0; k < KN; ++k) {
+            av[k] += __shfl_xor(av[k], m, 64);
+            ae[k] += __shfl_xor(ae[k], m, 64);
+            ak[k] += __shfl_xor(ak[k], m, 64);
+        }
+    if (live && sub == 0) {
         const size_t ob = ((size_t)f * G + gp) * OD * OD;
         auto store = [&](double *dst, const double *acc, double scale) {
             for (int aa = 0; aa < OD; ++aa) dst[ob + aa + OD * aa] = acc[aa] * scale;

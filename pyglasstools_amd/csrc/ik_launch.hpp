@@ -58,6 +58,17 @@ void launch_gather_2d32(const LaunchCtx &c, unsigned fields, int cg);  // ik_gat
 void launch_gather_3d2(const LaunchCtx &c, unsigned fields, int cg);   // ik_gather_3d2.hip
 void launch_gather_3d64(const LaunchCtx &c, unsigned fields, int cg);  // ik_gather_3d64.hip
 void launch_gather_3d32(const LaunchCtx &c, unsigned fields, int cg);  // ik_gather_3d32.hip
+// Lanes per grid point of the adaptive gather: the point's bond integrals
+// are dealt to them, so that a small grid still fills the GPU (about 64K
+// lanes per frame; a power of two from 1 to 32).  Measured on a 1,024-point
+// frame: 8 lanes 5.6 ms, 16 lanes 3.4 ms, 32 lanes 2.2 ms (Octic).
+inline int adaptive_lanes(int64_t G)
+{
+    int l = 1;
+    while (l < 32 && (int64_t)l * G < 65536) l <<= 1;
+    return l;
+}
+
 // Field gather with the adaptive (QAG) bond integral.  ws null: each
 // integral's interval table in the lane's private memory (kQagMax
 // intervals), and capped gets 1 when an integral needs more; else the global
