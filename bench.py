@@ -79,7 +79,8 @@ def host_cpu():
 
 
 def cpu_baseline(systems, grid, max_rcut, seconds=10.0):
-    """Oracle on the host: frames in order until `seconds` of wall time, on
+    """Oracle on the host: after a warm-up frame, frames in order until
+    `seconds` of wall time, the rate from the median per-frame time, on
     every CPU this process may use (the affinity mask; an OMP_NUM_THREADS set
     by the host environment caps it), built for this host's CPU
     (-march=native, as the reference's CompilerFlagsSetup.cmake:14 builds)."""
@@ -99,15 +100,21 @@ def cpu_baseline(systems, grid, max_rcut, seconds=10.0):
         build = "prebuilt -O3 -march=x86-64-v3 -fopenmp (native build failed: %s)" % exc
     out = {}
     for mode, name in ((1, "clean"), (0, "faithful")):
-        done, t0 = 0, time.perf_counter()
+        # one warm-up frame, then frames in order until `seconds`; the rate is
+        # one over the median per-frame time (the mean beside it)
+        oracle.ik_compute(systems[0], grid, fields=("virial", "kinetic"), max_rcut=max_rcut, mode=mode,
+                          nthreads=threads)
+        done, ts, t0 = 0, [], time.perf_counter()
         while done < len(systems):
+            t1 = time.perf_counter()
             oracle.ik_compute(systems[done], grid, fields=("virial", "kinetic"), max_rcut=max_rcut, mode=mode,
                               nthreads=threads)
+            ts.append(time.perf_counter() - t1)
             done += 1
             if time.perf_counter() - t0 >= seconds:
                 break
         dt = time.perf_counter() - t0
-        out[name] = {"frames": done, "seconds": dt, "fps": done / dt}
+        out[name] = {"frames": done, "seconds": dt, "fps": 1.0 / float(np.median(ts)), "fps_mean": done / dt}
     info["build"] = build
     info["threads"] = threads
     return out, threads, info
@@ -770,9 +777,9 @@ def main():
         res, threads, hinfo = cpu_baseline(systems, grid, max_rcut, args.cpu_seconds)
         log("cpu oracle: %s on %s" % (res, hinfo))
         cpu = {"value": round(res["clean"]["fps"], 4), "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": "%d of the %d frames, full 256x256 grid, FP64 oracle (C restatement of "
-                         "IrvingKirkwood::compute, OpenMP over grid points, no per-triple allocations)"
-                         % (res["clean"]["frames"], F),
+               "sample": "%d of the %d frames after one warm-up frame, full 256x256 grid, median per-frame "
+                         "time; FP64 oracle (C restatement of IrvingKirkwood::compute, OpenMP over grid points, "
+                         "no per-triple allocations)" % (res["clean"]["frames"], F),
                "cpu_model": hinfo["model"], "cpus_online": hinfo["cpus_online"],
                "cpus_usable": hinfo["cpus_usable"], "build": hinfo["build"]}
         extra["cpu_faithful_fps"] = res["faithful"]["fps"]
