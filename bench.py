@@ -448,14 +448,33 @@ def learned_surrogate(args):
 C5_N = (4096, 8192, 16384, 24576, 32768, 49152, 65536, 98304)
 
 
+def gs_nodes(graph):
+    """Nodes of a captured torch graph made with keep_graph=True, before it
+    is instantiated (hipGraphGetNodes; None where the runtime does not tell)."""
+    try:
+        import ctypes
+
+        hip = ctypes.CDLL("libamdhip64.so")
+        g = ctypes.c_void_p(graph.raw_cuda_graph())
+        n = ctypes.c_size_t(0)
+        if not g.value or hip.hipGraphGetNodes(g, None, ctypes.byref(n)) != 0:
+            return None
+        return int(n.value)
+    except Exception:
+        return None
+
+
 def batched_sweep(args):
     """Serving-style sweep: 1000 2D frames from 8 systems of uneven N (125
     frames each, seeds 5000 + 1000 k + f), each on a 128 x 128 grid over its
     own box, virial + kinetic.  One engine per system, launch shape tuned per
     system by the autotuner; then the whole sweep runs (a) eagerly, system
-    after system, and (b) as one HIP graph holding every system's launches
-    (pgt_ik_enqueue_frames captured on one stream), replayed.  Both for the
-    FP64 plain form and the FP32 form."""
+    after system, (b) as one HIP graph holding every system's launches
+    (pgt_ik_enqueue_frames captured on one stream), replayed, (c) the same
+    with the systems on streams forked inside the graph, and (d) as one
+    sweep (pgt_ik_sweep): one launch per stage over every system's frames,
+    eagerly and captured.  Both for the FP64 plain form and the FP32 form;
+    the line's value is (d) captured, FP64."""
     import torch
 
     from pyglasstools_amd import engine, workload
@@ -566,24 +585,68 @@ def batched_sweep(args):
         for j in jobs:
             j["e"].finish()
             j["e"].set_stream(0)
+        # (d) one sweep over every system (pgt_ik_sweep): one launch per stage
+        # for all 1000 frames, the tile gather at each engine's tuned cell
+        # reach; run once eagerly, then captured (about ten graph nodes)
+        outs = [(torch.empty_like(j["ov"]), torch.empty_like(j["ok"])) for j in jobs]
+        sw = engine.Sweep([dict(engine=j["e"], nframes=FR, n=j["sy"]["N"], pos=j["sy"]["pos"], diam=j["sy"]["diam"],
+                                mass=j["sy"]["mass"], vel=j["sy"]["vel"], virial=o[0], kinetic=o[1])
+                           for j, o in zip(jobs, outs)], fields=("virial", "kinetic"), stream_handle=s.cuda_stream)
+        sw.run()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sw.run()
+        t_sweep_eager = (time.perf_counter() - t0) / args.steps
+        sweep_stats = sw.stats()
+        # agreement with the engines' own results (the tile gather sums in the
+        # same order as the sweep; the lane gather, where an engine chose it,
+        # in another)
+        dmax = 0.0
+        for j, o in zip(jobs, outs):
+            for a, b in ((o[0], j["ov"]), (o[1], j["ok"])):
+                dmax = max(dmax, float((a - b).abs().max() / b.abs().max()))
+        gs = torch.cuda.CUDAGraph(keep_graph=True)
+        with torch.cuda.graph(gs, stream=s):
+            sw.enqueue()
+        nodes = gs_nodes(gs)
+        gs.instantiate()
+        sw.finish()
+        gs.replay()
+        torch.cuda.synchronize()
+        sw.finish()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            gs.replay()
+        torch.cuda.synchronize()
+        t_sweep = (time.perf_counter() - t0) / args.steps
+        sw.finish()
+        sw.release()
         nframes = FR * len(jobs)
         results[prec] = {"frames_per_s_graph": nframes / t_graph, "frames_per_s_eager": nframes / t_eager,
                          "frames_per_s_graph_%d_streams" % K: nframes / t_graph_k,
+                         "frames_per_s_sweep_graph": nframes / t_sweep,
+                         "frames_per_s_sweep_eager": nframes / t_sweep_eager,
+                         "sweep_graph_nodes": nodes,
+                         "sweep_gather_ms_eager": sweep_stats["gather_ms"],
+                         "sweep_total_ms_eager": sweep_stats["total_ms"],
+                         "sweep_max_rel_diff_vs_engines": dmax,
                          "contributions_per_s_graph": nc / args.steps / t_graph,
                          "tuning": {str(j["sy"]["N"]): j["tuning"] for j in jobs}}
-        log("%s sweep: graph %.1f frames/s (%d streams: %.1f), eager %.1f frames/s" %
-            (prec, nframes / t_graph, K, nframes / t_graph_k, nframes / t_eager))
-        del jobs, g, g2
+        log("%s sweep: per-engine graph %.1f frames/s (%d streams: %.1f), eager %.1f; one sweep: graph %.1f "
+            "(%s nodes), eager %.1f frames/s, max rel diff vs engines %.2e" %
+            (prec, nframes / t_graph, K, nframes / t_graph_k, nframes / t_eager, nframes / t_sweep, nodes,
+             nframes / t_sweep_eager, dmax))
+        del jobs, g, g2, gs, sw, outs
         torch.cuda.empty_cache()
-    best = results["fp64"]["frames_per_s_graph_%d_streams" % args.sweep_streams]
+    best = results["fp64"]["frames_per_s_sweep_graph"]
     line = {"metric": METRIC, "value": round(best, 3), "unit": "frames/s",
             "n_gpus": 1, "steps": args.steps, "warmup": 1,
             "ms_per_step": round(FR * len(C5_N) / best * 1e3, 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (seeded lattice+jitter polydisperse glasses, 8 systems x 125 frames resident in HBM)",
             "config": {"workload": "2D polydisperse12, 8 systems N in %s x 125 frames, 128x128 grid each, Octic "
-                                   "r_cg=2.0, GL5, virial + kinetic; one HIP graph per sweep, systems on %d streams forked "
-                                   "inside it" % (list(C5_N), args.sweep_streams),
+                                   "r_cg=2.0, GL5, virial + kinetic; one sweep (one launch per stage over all "
+                                   "systems' frames) captured in one HIP graph" % (list(C5_N),),
                        "frames_per_step": FR * len(C5_N), "grid": "128x128", "parallelism": "single GPU"},
             "roofline": None, "cpu_baseline": None,
             "extra": {"fp64": results["fp64"], "fp32_form": results["fp32"],

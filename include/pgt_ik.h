@@ -222,6 +222,67 @@ int pgt_ik_finish(pgt_ik_engine *e);
 /* Run the engine's work on the given hipStream_t (NULL: the engine's own). */
 int pgt_ik_set_stream(pgt_ik_engine *e, void *hip_stream);
 
+/* ---- sweep: many systems' frames, one launch per stage -------------------
+ * No counterpart in the reference, whose analyze loop
+ * (pyglasstools/__init__.py:119-122) computes one system's frames one call
+ * at a time.  A sweep computes the frames of several 2D systems (each its own
+ * engine: box, grid, potential, kernel radius, cell reach; particle counts may
+ * differ) with one launch per stage over all of them: cell-list passes, sorted
+ * arrays, bond list, field gather, counters (3 x passes + 4 launches, 7 or 10
+ * in all), so a captured graph holds about ten nodes whatever the number of
+ * systems.  Each system's fields equal its engine's pgt_ik_compute_frames
+ * with the tile gather, bit for bit.  Forms: 2D (positions and grid with
+ * z = 0), Octic kernel, fixed 5-node rule, FP64 or FP32 bond entries (the
+ * same for every system), virial and/or kinetic fields; others give
+ * PGT_ERR_UNSUPPORTED.  At create the sweep copies what it needs from the
+ * engines (geometry, parameters, grid), so they may change or be destroyed
+ * afterwards without affecting it; the device pointers in `systems` are kept
+ * and read by every run. */
+typedef struct pgt_ik_sweep pgt_ik_sweep;
+
+typedef struct pgt_ik_sweep_system {
+    pgt_ik_engine *engine;       /* box, grid and parameters set */
+    int32_t nframes;             /* >= 1 */
+    int64_t n;                   /* particles per frame, 1 .. 2^26 */
+    const double *d_pos;         /* device, nframes * n * 3 (z = 0) */
+    const double *d_diam;        /* device, nframes * n */
+    const double *d_mass;        /* device, nframes * n, or NULL (unit mass) */
+    const double *d_vel;         /* device, nframes * n * 3, or NULL (at rest) */
+    double *d_out_virial;        /* device, nframes * G * 4 (fields with virial) */
+    double *d_out_kinetic;       /* device, nframes * G * 4 (fields with kinetic) */
+} pgt_ik_sweep_system;
+
+/* Create a sweep over nsys (1 .. 16) systems on the current device, launching
+ * on hip_stream (NULL: a stream of its own).  fields: PGT_FIELD_VIRIAL and/or
+ * PGT_FIELD_KINETIC. */
+int pgt_ik_sweep_create(int nsys, const pgt_ik_sweep_system *systems, unsigned fields, void *hip_stream,
+                        pgt_ik_sweep **out);
+void pgt_ik_sweep_destroy(pgt_ik_sweep *s);
+
+/* Run the sweep and wait for it: sizes the work arrays and grows the bond
+ * capacity (running again) as pgt_ik_compute_frames does; sets the counts of
+ * pgt_ik_sweep_get_stats (totals over every system). */
+int pgt_ik_sweep_run(pgt_ik_sweep *s);
+
+/* Enqueue the sweep's launches and return: no synchronisation, allocation,
+ * memset or event, so they can be captured into a HIP graph on the sweep's
+ * stream.  Needs an earlier pgt_ik_sweep_run (PGT_ERR_STATE otherwise); check
+ * with pgt_ik_sweep_finish.  While launches may be replayed the sweep must
+ * not be destroyed and pgt_ik_sweep_run must not grow its buffers (it fails
+ * with PGT_ERR_STATE while held; pgt_ik_sweep_release declares the graph
+ * gone). */
+int pgt_ik_sweep_enqueue(pgt_ik_sweep *s);
+int pgt_ik_sweep_release(pgt_ik_sweep *s);
+
+/* Wait for the sweep's stream and check the work enqueued (or replayed) since
+ * the last finish: PGT_ERR_RANGE if a particle had more bonds than the
+ * capacity sized by the last run, or a tile item more than 255 bonds (the
+ * results are incomplete).  Sets the counts of pgt_ik_sweep_get_stats. */
+int pgt_ik_sweep_finish(pgt_ik_sweep *s);
+
+int pgt_ik_sweep_set_stream(pgt_ik_sweep *s, void *hip_stream);
+int pgt_ik_sweep_get_stats(pgt_ik_sweep *s, pgt_ik_stats *out);
+
 /* Launch shape: threads per workgroup of the field gather (128 or 256),
  * frames per launch, grid points per gather lane (1, 2 or 4: neighbouring
  * points that share one read of each particle and bond entry), and cell

@@ -42,7 +42,7 @@ def _run(cmd):
 # translation units of libpgt_ik.so, compiled in parallel: the host engine
 # with the cell-list, bond-list and surrogate kernels, and the field gathers
 ENGINE_SOURCES = ["ik_gather_3d64.hip", "ik_gather_3d32.hip", "ik_gather_3d2.hip", "ik_gather_2d64.hip",
-                  "ik_gather_2d32.hip", "ik_engine.hip", "ik_gather_adaptive.hip"]
+                  "ik_gather_2d32.hip", "ik_engine.hip", "ik_gather_adaptive.hip", "ik_sweep_gather.hip"]
 
 
 def build_engine():

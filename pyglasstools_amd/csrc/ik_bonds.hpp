@@ -55,32 +55,32 @@ struct BondLayout {
 // FP32 against bounds of the rounding (Geom::bin32, bout32), falling back to
 // the FP64 test in the band between them, so the list is that of the FP64
 // test; the kernel then stores each particle's bond count in its record.
+//
+// The body works on one frame's arrays (cell starts st, sorted positions sp,
+// diameters sd, displacements sdp, counts bc, entries bd, FP32 records rf)
+// for the particles of block pb (BLK consecutive cell-sorted particles), and
+// stores the wavefronts' largest counts at wmax[0 .. BLK/64).
 template <int D, int OD, bool VIR, bool ELA, typename ER, int BLK, int MAXC>
-__global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, double cgnorm,
-                                               const int *__restrict__ cstart, const double *__restrict__ spos,
-                                               const double *__restrict__ sdiam, const double *__restrict__ sdisp,
-                                               int cap, int *__restrict__ bcount, ER *__restrict__ bonds,
-                                               int *__restrict__ wave_max, float4 *__restrict__ rec)
+__device__ __forceinline__ void bonds_body(const Geom &g, int n, const PotParams &P, double cgnorm,
+                                           const int *__restrict__ st, const double *__restrict__ sp,
+                                           const double *__restrict__ sd, const double *__restrict__ sdp, int cap,
+                                           int *__restrict__ bc, ER *__restrict__ bd, int *__restrict__ wmax,
+                                           float4 *__restrict__ rf, int pb)
 {
     constexpr int PS = (D == 2) ? 2 : 4;
     using LY = BondLayout<D, OD, ELA, ER>;
     constexpr int ES = LY::ES;
     __shared__ int lq[MAXC][BLK];
     const int lane = threadIdx.x;
-    const int f = blockIdx.y;
-    const int p_raw = xcd_tile(blockIdx.x, gridDim.x) * BLK + lane;
+    const int p_raw = pb * BLK + lane;
     // lanes past the end shadow the last particle without writing, so the
     // whole wavefront takes part in the max-reduction at the end
     const bool live = p_raw < n;
     const int p = live ? p_raw : n - 1;
-    const double *sp = spos + (size_t)f * n * PS;
-    const double *sd = sdiam + (size_t)f * n;
-    const int *st = cstart + (size_t)f * (g.ncell + 1);
     const double xi = sp[(size_t)p * PS], yi = sp[(size_t)p * PS + 1], zi = (D == 3) ? sp[(size_t)p * PS + 2] : 0.0;
     const double di = sd[p];
-    const float4 *rf = D == 3 ? rec + (size_t)f * n : nullptr;  // 3D: FP32 records of this frame
     const float xi32 = (float)xi, yi32 = (float)yi, zi32 = (float)zi;
-    ER *out = bonds + ((size_t)f * n + p) * (size_t)cap * ES;
+    ER *out = bd + (size_t)p * (size_t)cap * ES;
     const double h = 0.5 * cgnorm;
     int cnt = 0;
     constexpr int NP = LY::GRP > 1 ? LY::GRP - 1 : 1;
@@ -137,7 +137,6 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
                 if constexpr (std::is_same<ER, double>::value) f0 = force_divr<double>(P, rsq, di, dj);
                 else f0 = (double)force_divr<float>(P, (float)rsq, (float)di, (float)dj);
                 if (ELA) {
-                    const double *sdp = sdisp + (size_t)f * n * 3;
                     for (int a = 0; a < 3; ++a) ur[a] = sdp[(size_t)q * 3 + a] - sdp[(size_t)p * 3 + a] + b[a];
                     const double ursq = ur[0] * ur[0] + ur[1] * ur[1] + ur[2] * ur[2];
                     if constexpr (std::is_same<ER, double>::value) f1 = force_divr<double>(P, ursq, di, dj);
@@ -275,17 +274,38 @@ __global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, doubl
     }
 
     if (live) {
-        bcount[(size_t)f * n + p] = cnt;
+        bc[p] = cnt;
         // 3D: the bond count completes the particle's FP32 record (the
         // candidate scans read only its position words)
-        if (D == 3) reinterpret_cast<int *>(rec + (size_t)f * n + p)[3] = cnt;
+        if (D == 3) reinterpret_cast<int *>(rf + p)[3] = cnt;
     }
     int mc = live ? cnt : 0;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) mc = max(mc, __shfl_xor(mc, off, 64));
     // the largest count of the wavefront, stored per wavefront in launch
     // order (k_reduce_stats takes the maximum)
-    if ((lane & 63) == 0) wave_max[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (BLK / 64) + (lane >> 6)] = mc;
+    if ((lane & 63) == 0) wmax[lane >> 6] = mc;
+}
+
+// One frame per y row; block x takes the particles of block
+// xcd_tile(x, gridDim.x), so each XCD works through a contiguous eighth of
+// the frame.
+template <int D, int OD, bool VIR, bool ELA, typename ER, int BLK, int MAXC>
+__global__ __launch_bounds__(BLK) void k_bonds(Geom g, int n, PotParams P, double cgnorm,
+                                               const int *__restrict__ cstart, const double *__restrict__ spos,
+                                               const double *__restrict__ sdiam, const double *__restrict__ sdisp,
+                                               int cap, int *__restrict__ bcount, ER *__restrict__ bonds,
+                                               int *__restrict__ wave_max, float4 *__restrict__ rec)
+{
+    constexpr int PS = (D == 2) ? 2 : 4;
+    constexpr int ES = BondLayout<D, OD, ELA, ER>::ES;
+    const int f = blockIdx.y;
+    const size_t fo = (size_t)f * n;
+    bonds_body<D, OD, VIR, ELA, ER, BLK, MAXC>(
+        g, n, P, cgnorm, cstart + (size_t)f * (g.ncell + 1), spos + fo * PS, sdiam + fo,
+        ELA ? sdisp + fo * 3 : nullptr, cap, bcount + fo, bonds + fo * (size_t)cap * ES,
+        wave_max + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (BLK / 64), D == 3 ? rec + fo : nullptr,
+        xcd_tile(blockIdx.x, gridDim.x));
 }
 
 }  // namespace pgt

@@ -35,6 +35,7 @@
 #include "ik_band.hpp"
 #include "ik_kernels.hpp"
 #include "ik_launch.hpp"
+#include "ik_sweep_kernels.hpp"
 #include "pgt_ik.h"
 
 using namespace pgt;
@@ -1734,6 +1735,412 @@ int pgt_ik_compute_surrogate(pgt_ik_engine *e, const float *d_params, double bou
     e->stats.n_pairs = np;
     e->stats.n_network = (int64_t)used;
     e->stats.surrogate_ms = pairs_ms;
+    return PGT_OK;
+}
+
+}  // extern "C"
+
+// ==========================================================================
+// sweep: the frames of several 2D systems, one launch per stage
+// ==========================================================================
+struct pgt_ik_sweep {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    unsigned fields = 0;
+    bool vir = false, kin = false, fp32 = false;
+    int ns = 0;
+    std::vector<SweepSys> tab;
+    GLRule R{};
+    int npass = 1, nbits = 1, radix = 2;
+    int64_t nframes = 0;   // frames of every system
+    int64_t np = 0;        // particles of every frame
+    int64_t npoints = 0;   // grid points of every frame
+    int64_t ncs = 0;       // cell starts of every frame
+    int64_t nh = 0;        // histogram entries of every frame
+    int blocks[SW_NSTAGE] = {0, 0, 0, 0};
+    size_t ebytes = 0;     // bytes of one bond entry
+    int cap = 0;
+    bool ready = false;    // buffers sized by a run
+    bool held = false;     // enqueued launches a captured graph may replay
+    DBuf<SweepSys> d_tab;
+    DBuf<double> d_grid;   // the systems' grids, back to back
+    DBuf<int> d_perm;      // and their Morton orders
+    DBuf<unsigned char> d_keys0, d_vals0, d_keys1, d_vals1, d_hist, d_cstart, d_spos, d_sdiam, d_bcount, d_skin;
+    DBuf<unsigned char> d_bonds, d_wcounts, d_wmax;
+    DBuf<int> d_maxcount;                // [0] largest bond count, [1] tile overflow bits
+    DBuf<unsigned long long> d_stats;    // balls, contributions
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    pgt_ik_stats stats{};
+};
+
+namespace {
+
+int check_sweep(pgt_ik_sweep *s)
+{
+    if (!s) return fail(PGT_ERR_ARG, "null sweep");
+    return 0;
+}
+
+int fail_sweep_held()
+{
+    return fail(PGT_ERR_STATE, "the sweep's buffers are held by enqueued work that a captured graph may replay: "
+                               "call pgt_ik_sweep_release first, and capture again afterwards");
+}
+
+// Work arrays of the sweep at its current bond capacity; grown unless held
+// (then PGT_ERR_STATE when too small).
+int sweep_buffers(pgt_ik_sweep *s)
+{
+    const size_t np = (size_t)s->np;
+    const size_t want[13] = {np * sizeof(int), np * sizeof(int), np * sizeof(int), np * sizeof(int),
+                             (size_t)s->nh * sizeof(int), (size_t)s->ncs * sizeof(int), np * 2 * sizeof(double),
+                             np * sizeof(double), s->vir ? np * sizeof(int) : 0, s->kin ? np * 3 * sizeof(double) : 0,
+                             s->vir ? np * (size_t)s->cap * s->ebytes : 0,
+                             (size_t)s->blocks[SW_GATHER] * 2 * sizeof(unsigned long long),
+                             s->vir ? (size_t)s->blocks[SW_BONDS] * 2 * sizeof(int) : 0};
+    DBuf<unsigned char> *bufs[13] = {&s->d_keys0, &s->d_vals0, &s->d_keys1, &s->d_vals1, &s->d_hist,
+                                     &s->d_cstart, &s->d_spos, &s->d_sdiam, &s->d_bcount, &s->d_skin,
+                                     &s->d_bonds, &s->d_wcounts, &s->d_wmax};
+    for (int k = 0; k < 13; ++k) {
+        if (!want[k] || (bufs[k]->p && bufs[k]->n >= want[k])) continue;
+        if (s->held) return fail_sweep_held();
+        if (int rc = bufs[k]->ensure(want[k])) return rc;
+    }
+    if (int rc = s->d_maxcount.ensure(2)) return rc;
+    return s->d_stats.ensure(2);
+}
+
+// Every launch of the sweep on its stream, no host synchronisation.  With
+// timed, events around the whole and around the gather.
+int sweep_launch(pgt_ik_sweep *s, bool timed)
+{
+    hipStream_t st = s->stream;
+    const SweepSys *T = s->d_tab.p;
+    const int ns = s->ns;
+    int *kb[2] = {(int *)s->d_keys0.p, (int *)s->d_keys1.p}, *vb[2] = {(int *)s->d_vals0.p, (int *)s->d_vals1.p};
+    int *hist = (int *)s->d_hist.p, *cstart = (int *)s->d_cstart.p, *bcount = (int *)s->d_bcount.p;
+    double *spos = (double *)s->d_spos.p, *sdiam = (double *)s->d_sdiam.p, *skin = (double *)s->d_skin.p;
+    if (timed) HIPCHK(hipEventRecord(s->ev[0], st));
+    const dim3 sg((unsigned)s->blocks[SW_SORT]);
+    for (int p = 0; p < s->npass; ++p) {
+        const int shift = p * s->nbits, radix = s->radix;
+        int *kin = kb[p & 1], *vin = vb[p & 1], *kout = kb[(p + 1) & 1], *vout = vb[(p + 1) & 1];
+        if (p == 0) {
+            hipLaunchKernelGGL(k_sweep_hist<true>, sg, dim3(kSortBlock), 0, st, T, ns, shift, radix, kin, hist);
+            hipLaunchKernelGGL(k_sweep_scan, dim3((unsigned)s->nframes), dim3(1024), 0, st, T, ns, radix, hist);
+            hipLaunchKernelGGL(k_sweep_scatter<true>, sg, dim3(kSortBlock), 0, st, T, ns, shift, radix, kin, vin,
+                               hist, kout, vout);
+        } else {
+            hipLaunchKernelGGL(k_sweep_hist<false>, sg, dim3(kSortBlock), 0, st, T, ns, shift, radix, kin, hist);
+            hipLaunchKernelGGL(k_sweep_scan, dim3((unsigned)s->nframes), dim3(1024), 0, st, T, ns, radix, hist);
+            hipLaunchKernelGGL(k_sweep_scatter<false>, sg, dim3(kSortBlock), 0, st, T, ns, shift, radix, kin, vin,
+                               hist, kout, vout);
+        }
+    }
+    hipLaunchKernelGGL(k_sweep_finish, dim3((unsigned)s->blocks[SW_FINISH]), dim3(256), 0, st, T, ns,
+                       kb[s->npass & 1], vb[s->npass & 1], cstart, spos, sdiam, s->kin ? skin : nullptr);
+    HIPCHK(hipGetLastError());
+    int *wmax = (int *)s->d_wmax.p;
+    if (s->vir) {
+        const dim3 bg((unsigned)s->blocks[SW_BONDS]);
+        if (s->fp32)
+            hipLaunchKernelGGL((k_sweep_bonds<float, kBondBlock, 32>), bg, dim3(kBondBlock), 0, st, T, ns, s->cap,
+                               cstart, spos, sdiam, bcount, (float *)s->d_bonds.p, wmax);
+        else
+            hipLaunchKernelGGL((k_sweep_bonds<double, kBondBlock, 32>), bg, dim3(kBondBlock), 0, st, T, ns, s->cap,
+                               cstart, spos, sdiam, bcount, (double *)s->d_bonds.p, wmax);
+        HIPCHK(hipGetLastError());
+    }
+    if (timed) HIPCHK(hipEventRecord(s->ev[1], st));
+    SweepGatherCtx c{};
+    c.tab = T;
+    c.ns = ns;
+    c.blocks = s->blocks[SW_GATHER];
+    c.R = s->R;
+    c.cstart = cstart;
+    c.spos = spos;
+    c.skin = s->kin ? skin : nullptr;
+    c.cap = s->cap;
+    c.bcount = s->vir ? bcount : nullptr;
+    c.bonds = s->vir ? (const void *)s->d_bonds.p : nullptr;
+    c.wave_counts = (unsigned long long *)s->d_wcounts.p;
+    c.flags = s->d_maxcount.p + 1;
+    c.stream = st;
+    launch_sweep_gather(c, s->fp32, s->fields);
+    HIPCHK(hipGetLastError());
+    if (timed) HIPCHK(hipEventRecord(s->ev[2], st));
+    const int nwg = s->blocks[SW_GATHER], nwb = s->vir ? s->blocks[SW_BONDS] * (kBondBlock / 64) : 0;
+    const int nrb = std::max(1, std::min(256, (std::max(nwg, nwb) + 4095) / 4096));
+    hipLaunchKernelGGL(k_reduce_stats, dim3(nrb), dim3(1024), 0, st, nwg, c.wave_counts, nwb, wmax, s->d_stats.p,
+                       s->d_maxcount.p);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int sweep_flags_error(int flags1)
+{
+    if (flags1 & 6)
+        return fail(PGT_ERR_RANGE, "a particle had more than 255 bonds, more than the tile gather holds: compute "
+                                   "these systems with their engines and the lane gather (pgt_ik_set_gather_mode)");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pgt_ik_sweep_create(int nsys, const pgt_ik_sweep_system *systems, unsigned fields, void *hip_stream,
+                        pgt_ik_sweep **out)
+{
+    if (!out) return fail(PGT_ERR_ARG, "null output handle");
+    *out = nullptr;
+    if (nsys < 1 || nsys > kSweepMaxSystems)
+        return fail(PGT_ERR_ARG, "a sweep holds 1 to " + std::to_string(kSweepMaxSystems) + " systems");
+    if (!systems) return fail(PGT_ERR_ARG, "null systems");
+    if (fields == 0 || (fields & ~(unsigned)(PGT_FIELD_VIRIAL | PGT_FIELD_KINETIC)))
+        return fail(fields & PGT_FIELD_ELASTIC ? PGT_ERR_UNSUPPORTED : PGT_ERR_ARG,
+                    "a sweep computes the virial and/or kinetic fields");
+    int ndev = 0, dev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PGT_ERR_HIP, "no HIP device available: the sweep runs on the GPU only");
+    HIPCHK(hipGetDevice(&dev));
+    const bool vir = fields & PGT_FIELD_VIRIAL, kin = fields & PGT_FIELD_KINETIC;
+    bool fp32 = false;
+    for (int k = 0; k < nsys; ++k) {
+        const pgt_ik_sweep_system &y = systems[k];
+        const pgt_ik_engine *e = y.engine;
+        const std::string at = "system " + std::to_string(k) + ": ";
+        if (!e) return fail(PGT_ERR_ARG, at + "null engine");
+        if (!e->box_set || !e->have_grid || e->G < 1)
+            return fail(PGT_ERR_STATE, at + "set_box and a nonempty set_grid must precede the sweep");
+        if (e->device != dev) return fail(PGT_ERR_ARG, at + "engine on another device");
+        if (e->prm.dim != 2 || !e->grid_z_flat)
+            return fail(PGT_ERR_UNSUPPORTED, at + "a sweep runs 2D systems (grid and positions with z = 0)");
+        if (e->prm.cg_kind != PGT_CG_OCTIC || e->prm.quad_mode != PGT_QUAD_FIXED || e->gl_s.size() != 5)
+            return fail(PGT_ERR_UNSUPPORTED, at + "a sweep runs the Octic kernel with the fixed 5-node rule");
+        const bool f32 = e->prm.precision == PGT_PREC_FP32;
+        if (k == 0) fp32 = f32;
+        else if (f32 != fp32) return fail(PGT_ERR_ARG, at + "every system of a sweep has the same precision");
+        if (y.nframes < 1) return fail(PGT_ERR_ARG, at + "at least one frame");
+        if (y.n < 1 || y.n > kTileMaxParticles)
+            return fail(PGT_ERR_RANGE, at + "particles per frame must be 1 .. 2^26");
+        if (!y.d_pos || !y.d_diam) return fail(PGT_ERR_ARG, at + "positions and diameters are required");
+        if ((vir && !y.d_out_virial) || (kin && !y.d_out_kinetic))
+            return fail(PGT_ERR_ARG, at + "null output for a selected field");
+    }
+    auto *s = new pgt_ik_sweep();
+    auto bail = [&](int rc) {
+        pgt_ik_sweep_destroy(s);
+        return rc;
+    };
+    s->device = dev;
+    s->fields = fields;
+    s->vir = vir;
+    s->kin = kin;
+    s->fp32 = fp32;
+    s->ns = nsys;
+    s->R = engine_rule(systems[0].engine);
+    s->ebytes = fp32 ? sizeof(float) * BondLayout<2, 2, false, float>::ES
+                     : sizeof(double) * BondLayout<2, 2, false, double>::ES;
+    s->tab.resize((size_t)nsys);
+    int maxcell = 1, cap = 0;
+    int64_t gtotal = 0;
+    for (int k = 0; k < nsys; ++k) {
+        const pgt_ik_engine *e = systems[k].engine;
+        SweepSys &S = s->tab[(size_t)k];
+        S = SweepSys{};
+        S.g = make_geom(e, 2);
+        S.P = make_pot(e->prm);
+        S.cgnorm = cg_norm(e->prm.cg_kind, e->prm.cg_rcut);
+        S.n = (int)systems[k].n;
+        S.G = (int)e->G;
+        S.nf = systems[k].nframes;
+        S.nchunk = (S.n + kSortChunk - 1) / kSortChunk;
+        maxcell = std::max(maxcell, S.g.ncell);
+        cap = std::max(cap, e->cap);
+        gtotal += e->G;
+    }
+    int kbits = 0;
+    while ((1 << kbits) < maxcell) ++kbits;
+    s->npass = std::max(1, (kbits + 7) / 8);
+    s->nbits = std::max(1, (kbits + s->npass - 1) / s->npass);
+    s->radix = 1 << s->nbits;
+    s->cap = cap > 0 ? cap : 16;
+    // offsets and workgroup ranges, systems in order
+    int64_t blk[SW_NSTAGE] = {0, 0, 0, 0};
+    for (int k = 0; k < nsys; ++k) {
+        SweepSys &S = s->tab[(size_t)k];
+        S.f0 = s->nframes;
+        S.p0 = s->np;
+        S.c0 = s->ncs;
+        S.h0 = s->nh;
+        s->nframes += S.nf;
+        s->np += (int64_t)S.nf * S.n;
+        s->npoints += (int64_t)S.nf * S.G;
+        s->ncs += (int64_t)S.nf * (S.g.ncell + 1);
+        s->nh += (int64_t)S.nf * s->radix * S.nchunk;
+        S.bpf[SW_SORT] = S.nchunk;
+        S.bpf[SW_FINISH] = (S.n + 255) / 256;
+        S.bpf[SW_BONDS] = (S.n + kBondBlock - 1) / kBondBlock;
+        S.bpf[SW_GATHER] = (S.G + kTilePoints - 1) / kTilePoints;
+        for (int st = 0; st < SW_NSTAGE; ++st) {
+            const int64_t nb = (st == SW_SORT || st == SW_FINISH)
+                                   ? (int64_t)frame_rows(S.nf) * S.bpf[st]
+                                   : ((int64_t)S.nf * S.bpf[st] + 7) / 8 * 8;
+            S.blk0[st] = (int)blk[st];
+            S.nblk[st] = (int)nb;
+            blk[st] += nb;
+            if (blk[st] > (int64_t)1 << 24)
+                return bail(fail(PGT_ERR_RANGE, "the sweep's launches exceed 2^24 workgroups: split the sweep"));
+        }
+        S.pos = systems[k].d_pos;
+        S.diam = systems[k].d_diam;
+        S.mass = systems[k].d_mass;
+        S.vel = systems[k].d_vel;
+        S.vir = vir ? systems[k].d_out_virial : nullptr;
+        S.kin = kin ? systems[k].d_out_kinetic : nullptr;
+    }
+    for (int st = 0; st < SW_NSTAGE; ++st) s->blocks[st] = (int)blk[st];
+    if (s->np > ((int64_t)1 << 31) - 1)
+        return bail(fail(PGT_ERR_RANGE, "the sweep holds more than 2^31 - 1 particles over all frames: split it"));
+    int rc;
+    if ((rc = s->d_grid.ensure((size_t)gtotal * 3)) || (rc = s->d_perm.ensure((size_t)gtotal)) ||
+        (rc = s->d_tab.ensure((size_t)nsys)))
+        return bail(rc);
+    int64_t goff = 0;
+    for (int k = 0; k < nsys; ++k) {
+        const pgt_ik_engine *e = systems[k].engine;
+        SweepSys &S = s->tab[(size_t)k];
+        S.grid = s->d_grid.p + goff * 3;
+        S.perm = s->d_perm.p + goff;
+        if (hipMemcpy((void *)S.grid, e->d_grid.p, sizeof(double) * 3 * (size_t)e->G, hipMemcpyDeviceToDevice) !=
+                hipSuccess ||
+            hipMemcpy((void *)S.perm, e->d_perm.p, sizeof(int) * (size_t)e->G, hipMemcpyDeviceToDevice) != hipSuccess)
+            return bail(fail(PGT_ERR_HIP, "copying a system's grid"));
+        goff += e->G;
+    }
+    if (hipMemcpy(s->d_tab.p, s->tab.data(), sizeof(SweepSys) * (size_t)nsys, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(PGT_ERR_HIP, "uploading the sweep's table"));
+    if (hipStreamCreate(&s->own_stream) != hipSuccess) return bail(fail(PGT_ERR_HIP, "hipStreamCreate"));
+    s->stream = hip_stream ? (hipStream_t)hip_stream : s->own_stream;
+    for (auto &v : s->ev)
+        if (hipEventCreate(&v) != hipSuccess) return bail(fail(PGT_ERR_HIP, "hipEventCreate"));
+    *out = s;
+    return PGT_OK;
+}
+
+void pgt_ik_sweep_destroy(pgt_ik_sweep *s)
+{
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    DBuf<unsigned char> *bufs[] = {&s->d_keys0, &s->d_vals0, &s->d_keys1, &s->d_vals1, &s->d_hist,
+                                   &s->d_cstart, &s->d_spos, &s->d_sdiam, &s->d_bcount, &s->d_skin,
+                                   &s->d_bonds, &s->d_wcounts, &s->d_wmax};
+    for (auto *b : bufs) b->release();
+    s->d_tab.release();
+    s->d_grid.release();
+    s->d_perm.release();
+    s->d_maxcount.release();
+    s->d_stats.release();
+    for (hipEvent_t v : s->ev)
+        if (v) (void)hipEventDestroy(v);
+    if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
+    delete s;
+}
+
+int pgt_ik_sweep_run(pgt_ik_sweep *s)
+{
+    if (int rc = check_sweep(s)) return rc;
+    int mx = 0;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        int rc;
+        if ((rc = sweep_buffers(s))) return rc;
+        if ((rc = fill_async(s->d_maxcount.p, 0, 2, s->stream)) || (rc = fill_async(s->d_stats.p, 0ull, 2, s->stream)))
+            return rc;
+        if ((rc = sweep_launch(s, true))) return rc;
+        int flags[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(flags, s->d_maxcount.p, sizeof(flags), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        if ((rc = sweep_flags_error(flags[1]))) return rc;
+        mx = flags[0];
+        if (!(s->vir && mx > s->cap)) break;
+        if (mx > 65535) return fail(PGT_ERR_RANGE, "a particle has more than 65535 interacting neighbours");
+        if (s->held) return fail_sweep_held();  // the graph's launches keep the old capacity
+        s->cap = ((mx + 7) / 8) * 8;
+        if (attempt == 3) return fail(PGT_ERR_HIP, "bond list capacity did not settle");
+    }
+    unsigned long long h[2] = {0, 0};
+    HIPCHK(hipMemcpy(h, s->d_stats.p, sizeof(h), hipMemcpyDeviceToHost));
+    float g_ms = 0.0f, tot = 0.0f;
+    (void)hipEventElapsedTime(&g_ms, s->ev[1], s->ev[2]);
+    (void)hipEventElapsedTime(&tot, s->ev[0], s->ev[2]);
+    s->stats = pgt_ik_stats{};
+    s->stats.n_particles = s->np;
+    s->stats.n_grid = s->npoints;
+    s->stats.n_ball = (int64_t)h[0];
+    s->stats.n_contrib = (int64_t)h[1];
+    s->stats.bond_capacity = s->cap;
+    s->stats.max_bonds = mx;
+    s->stats.gather_ms = g_ms;
+    s->stats.total_ms = tot;
+    s->ready = true;
+    return PGT_OK;
+}
+
+int pgt_ik_sweep_enqueue(pgt_ik_sweep *s)
+{
+    if (int rc = check_sweep(s)) return rc;
+    if (!s->ready) return fail(PGT_ERR_STATE, "run pgt_ik_sweep_run before enqueueing the sweep");
+    if (int rc = sweep_launch(s, false)) return rc;
+    s->held = true;
+    return PGT_OK;
+}
+
+int pgt_ik_sweep_release(pgt_ik_sweep *s)
+{
+    if (int rc = check_sweep(s)) return rc;
+    s->held = false;
+    return PGT_OK;
+}
+
+int pgt_ik_sweep_finish(pgt_ik_sweep *s)
+{
+    if (int rc = check_sweep(s)) return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (!s->ready) return PGT_OK;
+    int flags[2] = {0, 0};
+    unsigned long long h[2] = {0, 0};
+    HIPCHK(hipMemcpy(flags, s->d_maxcount.p, sizeof(flags), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h, s->d_stats.p, sizeof(h), hipMemcpyDeviceToHost));
+    int rc = fill_async(s->d_maxcount.p, 0, 2, s->stream);
+    if (!rc) rc = fill_async(s->d_stats.p, 0ull, 2, s->stream);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->stats.n_ball = (int64_t)h[0];
+    s->stats.n_contrib = (int64_t)h[1];
+    s->stats.max_bonds = flags[0];
+    s->stats.bond_capacity = s->cap;
+    if ((rc = sweep_flags_error(flags[1]))) return rc;
+    if (s->vir && flags[0] > s->cap)
+        return fail(PGT_ERR_RANGE, "a particle had more bonds than the capacity sized by the last "
+                                   "pgt_ik_sweep_run: the enqueued results are incomplete; release, run and "
+                                   "capture again");
+    return PGT_OK;
+}
+
+int pgt_ik_sweep_set_stream(pgt_ik_sweep *s, void *hip_stream)
+{
+    if (int rc = check_sweep(s)) return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->stream = hip_stream ? (hipStream_t)hip_stream : s->own_stream;
+    return PGT_OK;
+}
+
+int pgt_ik_sweep_get_stats(pgt_ik_sweep *s, pgt_ik_stats *out)
+{
+    if (int rc = check_sweep(s)) return rc;
+    if (!out) return fail(PGT_ERR_ARG, "null stats");
+    *out = s->stats;
     return PGT_OK;
 }
 

@@ -30,8 +30,10 @@
 
 #include "ik_bonds.hpp"
 #include "ik_device.hpp"
+#include "ik_gather.hpp"
 #include "ik_geom.hpp"
 #include "ik_launch.hpp"
+#include "ik_sweep.hpp"
 
 namespace pgt {
 
@@ -63,12 +65,19 @@ struct TileWaves {
     static constexpr int value = (D == 2 && !ELA) ? 4 : 1;
 };
 
+// The body works on one frame's arrays (cell starts st, sorted positions
+// sp, kinetic terms sk, bond counts bc, FP32 records rf, entries bd; out's
+// field pointers at the frame's first value) for the tile-th 64 points of
+// the Morton order perm, and stores its counts at out.wave_counts[2 wi ..].
 template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, bool KIN, typename ER>
-__global__ __launch_bounds__(64, (TileWaves<D, ELA>::value)) void k_gather_tile(
-    Geom g, int n, int G, const double *__restrict__ grid, const int *__restrict__ perm, GLRule R,
-    const double *__restrict__ gld, double cgnorm, const int *__restrict__ cstart, const double *__restrict__ spos,
-    const double *__restrict__ skin, int cap, const int *__restrict__ bcount, const float4 *__restrict__ rec,
-    const ER *__restrict__ bonds, GatherOut out, int *__restrict__ overflow)
+__device__ __forceinline__ void gather_tile_body(const Geom &g, int n, int G, const double *__restrict__ grid,
+                                                 const int *__restrict__ perm, const GLRule &R,
+                                                 const double *__restrict__ gld, double cgnorm,
+                                                 const int *__restrict__ st, const double *__restrict__ sp,
+                                                 const double *__restrict__ sk, int cap,
+                                                 const int *__restrict__ bc, const float4 *__restrict__ rf,
+                                                 const ER *__restrict__ bd, const GatherOut &out, size_t wi, int tile,
+                                                 int *__restrict__ overflow)
 {
     using CP = TileCaps;
     using LY = BondLayout<D, OD, ELA, ER>;
@@ -93,20 +102,13 @@ __global__ __launch_bounds__(64, (TileWaves<D, ELA>::value)) void k_gather_tile(
     __shared__ unsigned short s_perm[MAXI];   // the items in bond-count order
     __shared__ int s_bucket[64];
 
-    const int f = blockIdx.y;
     const int l = threadIdx.x;
-    const int t0 = xcd_tile(blockIdx.x, gridDim.x) * 64;
+    const int t0 = tile * 64;
     const int t = t0 + l;
     const bool live = t < G;
-    const double *sp = spos + (size_t)f * n * PS;
-    const int *st = cstart + (size_t)f * (g.ncell + 1);
-    const int *bc = bcount + (size_t)f * n;
-    const ER *bd = bonds + (size_t)f * n * (size_t)cap * ES;
-    const double *sk = KIN ? skin + (size_t)f * n * KN : nullptr;
     // 3D with bonds: the scan reads the bond kernel's 16-byte records (FP32
     // position, bond count) and tests in FP32 first, as the lane gather does
     constexpr bool RECS = PAIRS && D == 3;
-    const float4 *rf = RECS ? rec + (size_t)f * n : nullptr;
 
     const int gp = perm[live ? t : t0];
     double xg[3];
@@ -452,7 +454,7 @@ __global__ __launch_bounds__(64, (TileWaves<D, ELA>::value)) void k_gather_tile(
     __syncthreads();
 
     if (live) {
-        const size_t ob = ((size_t)f * G + gp) * OD * OD;
+        const size_t ob = (size_t)gp * OD * OD;
         double av[KN], ae[KN], ak[KN];
 #pragma unroll
         for (int k = 0; k < KN; ++k) {
@@ -477,11 +479,68 @@ __global__ __launch_bounds__(64, (TileWaves<D, ELA>::value)) void k_gather_tile(
     }
     ncontrib = wave_sum(ncontrib);
     if (l == 0) {
-        const size_t wi = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
         out.wave_counts[2 * wi] = nball;  // uniform: not summed over lanes
         out.wave_counts[2 * wi + 1] = ncontrib;
     }
     if (over) atomicOr(overflow, 2);
+}
+
+// One frame per y row; block x takes the tile xcd_tile(x, gridDim.x), so
+// each XCD works through a contiguous eighth of the frame's points.
+template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, bool KIN, typename ER>
+__global__ __launch_bounds__(64, (TileWaves<D, ELA>::value)) void k_gather_tile(
+    Geom g, int n, int G, const double *__restrict__ grid, const int *__restrict__ perm, GLRule R,
+    const double *__restrict__ gld, double cgnorm, const int *__restrict__ cstart, const double *__restrict__ spos,
+    const double *__restrict__ skin, int cap, const int *__restrict__ bcount, const float4 *__restrict__ rec,
+    const ER *__restrict__ bonds, GatherOut out, int *__restrict__ overflow)
+{
+    constexpr int PS = (D == 2) ? 2 : 4;
+    constexpr int KN = OD * (OD + 1) / 2;
+    constexpr int ES = BondLayout<D, OD, ELA, ER>::ES;
+    constexpr bool RECS = (VIR || ELA) && D == 3;
+    const int f = blockIdx.y;
+    const size_t fo = (size_t)f * n, oo = (size_t)f * G * OD * OD;
+    GatherOut of = out;
+    if (VIR) of.vir = out.vir + oo;
+    if (ELA) of.ela = out.ela + oo;
+    if (KIN) of.kin = out.kin + oo;
+    gather_tile_body<D, OD, CG, NQ, VIR, ELA, KIN, ER>(
+        g, n, G, grid, perm, R, gld, cgnorm, cstart + (size_t)f * (g.ncell + 1), spos + fo * PS,
+        KIN ? skin + fo * KN : nullptr, cap, bcount + fo, RECS ? rec + fo : nullptr, bonds + fo * (size_t)cap * ES, of,
+        (size_t)blockIdx.y * gridDim.x + blockIdx.x, xcd_tile(blockIdx.x, gridDim.x), overflow);
+}
+
+// The sweep's field gather (2D, virial and/or kinetic): each workgroup finds
+// its system, frame and tile in the sweep's table (ik_sweep.hpp).  Padding
+// workgroups store zero counts for their slot, so k_reduce_stats can read
+// every slot.
+template <bool VIR, bool KIN, typename ER>
+__global__ __launch_bounds__(64, (TileWaves<2, false>::value)) void k_sweep_gather(
+    const SweepSys *__restrict__ T, int ns, GLRule R, const int *__restrict__ cstart,
+    const double *__restrict__ spos, const double *__restrict__ skin, int cap, const int *__restrict__ bcount,
+    const ER *__restrict__ bonds, unsigned long long *__restrict__ wave_counts, int *__restrict__ overflow)
+{
+    constexpr int ES = BondLayout<2, 2, false, ER>::ES;
+    const SweepSys &S = T[sweep_system(T, ns, SW_GATHER, blockIdx.x)];
+    int f, tile;
+    if (!sweep_frame_tile(S, SW_GATHER, blockIdx.x, f, tile)) {
+        if (threadIdx.x == 0) {
+            wave_counts[2 * (size_t)blockIdx.x] = 0;
+            wave_counts[2 * (size_t)blockIdx.x + 1] = 0;
+        }
+        return;
+    }
+    const Geom &g = S.g;
+    const int n = S.n, G = S.G;
+    const size_t fo = (size_t)S.p0 + (size_t)f * n, oo = (size_t)f * G * 4;
+    GatherOut of{};
+    of.vir = VIR ? S.vir + oo : nullptr;
+    of.kin = KIN ? S.kin + oo : nullptr;
+    of.wave_counts = wave_counts;
+    gather_tile_body<2, 2, 0, 5, VIR, false, KIN, ER>(
+        g, n, G, S.grid, S.perm, R, nullptr, S.cgnorm, cstart + (size_t)S.c0 + (size_t)f * (g.ncell + 1),
+        spos + fo * 2, KIN ? skin + fo * 3 : nullptr, cap, VIR ? bcount + fo : nullptr, nullptr,
+        VIR ? bonds + fo * (size_t)cap * ES : nullptr, of, blockIdx.x, tile, overflow);
 }
 
 }  // namespace pgt

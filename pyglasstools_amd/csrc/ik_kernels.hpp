@@ -35,28 +35,26 @@ __device__ __forceinline__ int cell_of(const Geom &g, const double *p)
     return (cz * g.nc[1] + cy) * g.nc[0] + cx;
 }
 
-// Count the digit (key >> shift) & (radix - 1) over one chunk of one frame
-// into hist[f][digit][chunk].  With FIRST the keys are the particles' cells,
-// computed here from the input positions and stored in keys.
+// Count the digit (key >> shift) & (radix - 1) over chunk ch of one frame
+// into hf[digit * nchunk + ch] (hf: the frame's histogram).  With FIRST the
+// keys are the particles' cells, computed here from the frame's input
+// positions and stored in kf.
 template <bool FIRST>
-__global__ __launch_bounds__(kSortBlock) void k_sort_hist(Geom g, int n, int nf, int shift, int radix,
-                                                          const double *__restrict__ pos, int *__restrict__ keys,
-                                                          int *__restrict__ hist)
+__device__ __forceinline__ void sort_hist_body(const Geom &g, int n, int ch, int nchunk, int shift, int radix,
+                                               const double *__restrict__ pos, int *__restrict__ kf,
+                                               int *__restrict__ hf)
 {
     __shared__ int h[kSortMaxRadix];
-    int f, ch;
-    if (!frame_block(nf, f, ch)) return;
     const int t = threadIdx.x;
     h[t] = 0;
     __syncthreads();
-    int *kf = keys + (size_t)f * n;
     const int base = ch * kSortChunk;
     for (int r = 0; r < kSortRows; ++r) {
         const int i = base + r * kSortBlock + t;
         if (i >= n) break;
         int k;
         if constexpr (FIRST) {
-            k = cell_of(g, pos + ((size_t)f * n + i) * 3);
+            k = cell_of(g, pos + (size_t)i * 3);
             kf[i] = k;
         } else {
             k = kf[i];
@@ -64,15 +62,27 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_hist(Geom g, int n, int nf,
         atomicAdd(&h[(k >> shift) & (radix - 1)], 1);
     }
     __syncthreads();
-    if (t < radix) hist[((size_t)f * radix + t) * gridDim.x + ch] = h[t];
+    if (t < radix) hf[(size_t)t * nchunk + ch] = h[t];
 }
 
-// In-place exclusive scan of len ints per frame, one 1024-thread workgroup
-// per frame: hist[f][digit][chunk] becomes the first output slot of that
-// digit's items of that chunk.
-__global__ __launch_bounds__(1024) void k_sort_scan(int len, int *__restrict__ hist)
+// One frame per y row (frames dealt to XCDs whole), one chunk per workgroup:
+// hist[f][digit][chunk].
+template <bool FIRST>
+__global__ __launch_bounds__(kSortBlock) void k_sort_hist(Geom g, int n, int nf, int shift, int radix,
+                                                          const double *__restrict__ pos, int *__restrict__ keys,
+                                                          int *__restrict__ hist)
 {
-    int *a = hist + (size_t)blockIdx.x * len;
+    int f, ch;
+    if (!frame_block(nf, f, ch)) return;
+    sort_hist_body<FIRST>(g, n, ch, gridDim.x, shift, radix, pos + (size_t)f * n * 3, keys + (size_t)f * n,
+                          hist + (size_t)f * radix * gridDim.x);
+}
+
+// In-place exclusive scan of the len ints at a, by one 1024-thread
+// workgroup: a frame's hist[digit][chunk] becomes the first output slot of
+// that digit's items of that chunk.
+__device__ __forceinline__ void scan_body(int len, int *__restrict__ a)
+{
     const int T = blockDim.x, t = threadIdx.x;
     const int chunk = (len + T - 1) / T;
     const int b0 = min(len, t * chunk), b1 = min(len, b0 + chunk);
@@ -95,6 +105,12 @@ __global__ __launch_bounds__(1024) void k_sort_scan(int len, int *__restrict__ h
     }
 }
 
+// One workgroup per frame (blockIdx.x), len ints each.
+__global__ __launch_bounds__(1024) void k_sort_scan(int len, int *__restrict__ hist)
+{
+    scan_body(len, hist + (size_t)blockIdx.x * len);
+}
+
 // Move the items of one chunk to their slots for this digit.  A chunk is
 // taken row by row (kSortBlock consecutive items); inside a row an item's
 // slot is the chunk's offset for its digit, plus the items of that digit in
@@ -102,19 +118,16 @@ __global__ __launch_bounds__(1024) void k_sort_scan(int len, int *__restrict__ h
 // wavefront (found with one ballot per digit bit).  With FIRST the value is
 // the input index.
 template <bool FIRST>
-__global__ __launch_bounds__(kSortBlock) void k_sort_scatter(int n, int nf, int shift, int radix,
-                                                             const int *__restrict__ kin, const int *__restrict__ vin,
-                                                             const int *__restrict__ hist, int *__restrict__ kout,
-                                                             int *__restrict__ vout)
+__device__ __forceinline__ void sort_scatter_body(int n, int ch, int nchunk, int shift, int radix,
+                                                  const int *__restrict__ kin, const int *__restrict__ vin,
+                                                  const int *__restrict__ hf, int *__restrict__ kout,
+                                                  int *__restrict__ vout)
 {
     constexpr int NW = kSortBlock / 64;
     __shared__ int run[kSortMaxRadix];     // next slot of each digit
     __shared__ int wc[NW][kSortMaxRadix];  // per-wavefront digit counts of a row, then their slots
-    int f, ch;
-    if (!frame_block(nf, f, ch)) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const size_t fo = (size_t)f * n;
-    if (t < radix) run[t] = hist[((size_t)f * radix + t) * gridDim.x + ch];
+    if (t < radix) run[t] = hf[(size_t)t * nchunk + ch];
     const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;  // lanes under this one
     int nbits = 0;
     while ((1 << nbits) < radix) ++nbits;
@@ -127,8 +140,8 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(int n, int nf, int 
         const bool valid = i < n;
         int key = 0, val = 0, d = 0;
         if (valid) {
-            key = kin[fo + i];
-            val = FIRST ? i : vin[fo + i];
+            key = kin[i];
+            val = FIRST ? i : vin[i];
             d = (key >> shift) & (radix - 1);
         }
         unsigned long long peers = __ballot(valid);
@@ -152,11 +165,24 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(int n, int nf, int 
         __syncthreads();
         if (valid) {
             const int slot = wc[w][d] + rank;
-            kout[fo + slot] = key;
-            vout[fo + slot] = val;
+            kout[slot] = key;
+            vout[slot] = val;
         }
         __syncthreads();
     }
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(kSortBlock) void k_sort_scatter(int n, int nf, int shift, int radix,
+                                                             const int *__restrict__ kin, const int *__restrict__ vin,
+                                                             const int *__restrict__ hist, int *__restrict__ kout,
+                                                             int *__restrict__ vout)
+{
+    int f, ch;
+    if (!frame_block(nf, f, ch)) return;
+    const size_t fo = (size_t)f * n;
+    sort_scatter_body<FIRST>(n, ch, gridDim.x, shift, radix, kin + fo, vin + fo, hist + (size_t)f * radix * gridDim.x,
+                             kout + fo, vout + fo);
 }
 
 // Write the cell-sorted particle arrays (slot k takes input particle
@@ -167,6 +193,50 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(int n, int nf, int 
 // bit, inside the box), so a particle sits where its cell says.  In 3D with
 // bonds it also writes each particle's FP32 record (x, y, z; the bond kernel
 // adds the count) for the candidate scans' FP32 pre-tests.
+template <int D, int OD>
+__device__ __forceinline__ void sort_finish_body(const Geom &g, int n, int k, int ncell, const int *__restrict__ keys,
+                                                 const int *__restrict__ vals, const double *__restrict__ pos,
+                                                 const double *__restrict__ diam, const double *__restrict__ mass,
+                                                 const double *__restrict__ vel, const double *__restrict__ disp,
+                                                 int *__restrict__ st, double *__restrict__ spos,
+                                                 double *__restrict__ sdiam, double *__restrict__ skin,
+                                                 double *__restrict__ sdisp, float4 *__restrict__ rec)
+{
+    constexpr int PS = (D == 2) ? 2 : 4;
+    constexpr int KN = OD * (OD + 1) / 2;
+    if (k >= n) return;
+    const int c = keys[k];
+    const int cp = k == 0 ? -1 : keys[k - 1];
+    for (int q = cp + 1; q <= c; ++q) st[q] = k;
+    if (k == n - 1)
+        for (int q = c + 1; q <= ncell; ++q) st[q] = n;
+    const size_t fi = (size_t)vals[k];
+    const double *p = pos + fi * 3;
+    double *sp = spos + (size_t)k * PS;
+    auto wrap = [&](int a) {
+        return g.periodic[a] ? p[a] - g.L[a] * floor((p[a] - g.lo[a]) * g.invL[a]) : p[a];
+    };
+    sp[0] = wrap(0);
+    sp[1] = wrap(1);
+    if (D == 3) { sp[2] = wrap(2); sp[3] = 0.0; }
+    // 3D with bonds: the FP32 record of the candidate scans (its last word,
+    // the bond count, is filled in by the bond kernel)
+    if (D == 3 && rec) rec[k] = make_float4((float)sp[0], (float)sp[1], (float)sp[2], 0.0f);
+    sdiam[k] = diam[fi];
+    if (skin) {
+        const double m = mass ? mass[fi] : 1.0;
+        double v[3] = {0, 0, 0};
+        if (vel) { v[0] = vel[fi * 3]; v[1] = vel[fi * 3 + 1]; v[2] = vel[fi * 3 + 2]; }
+        double *kk = skin + (size_t)k * KN;
+        // symmetric packing: diagonal, then (0,1) [, (0,2), (1,2)]
+        for (int a = 0; a < OD; ++a) kk[a] = m * v[a] * v[a];
+        if (OD == 2) kk[2] = m * v[0] * v[1];
+        else { kk[3] = m * v[0] * v[1]; kk[4] = m * v[0] * v[2]; kk[5] = m * v[1] * v[2]; }
+    }
+    if (sdisp)
+        for (int a = 0; a < 3; ++a) sdisp[(size_t)k * 3 + a] = disp ? disp[fi * 3 + a] : 0.0;
+}
+
 template <int D, int OD>
 __global__ __launch_bounds__(256) void k_sort_finish(Geom g, int n, int nf, int ncell, const int *__restrict__ keys,
                                                      const int *__restrict__ vals, const double *__restrict__ pos,
@@ -180,40 +250,12 @@ __global__ __launch_bounds__(256) void k_sort_finish(Geom g, int n, int nf, int 
     constexpr int KN = OD * (OD + 1) / 2;
     int f, bx;
     if (!frame_block(nf, f, bx)) return;
-    const int k = bx * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const size_t fk = (size_t)f * n + k;
-    const int c = keys[fk];
-    const int cp = k == 0 ? -1 : keys[fk - 1];
-    int *st = cstart + (size_t)f * (ncell + 1);
-    for (int q = cp + 1; q <= c; ++q) st[q] = k;
-    if (k == n - 1)
-        for (int q = c + 1; q <= ncell; ++q) st[q] = n;
-    const size_t fi = (size_t)f * n + vals[fk];
-    const double *p = pos + fi * 3;
-    double *sp = spos + fk * PS;
-    auto wrap = [&](int a) {
-        return g.periodic[a] ? p[a] - g.L[a] * floor((p[a] - g.lo[a]) * g.invL[a]) : p[a];
-    };
-    sp[0] = wrap(0);
-    sp[1] = wrap(1);
-    if (D == 3) { sp[2] = wrap(2); sp[3] = 0.0; }
-    // 3D with bonds: the FP32 record of the candidate scans (its last word,
-    // the bond count, is filled in by the bond kernel)
-    if (D == 3 && rec) rec[fk] = make_float4((float)sp[0], (float)sp[1], (float)sp[2], 0.0f);
-    sdiam[fk] = diam[fi];
-    if (skin) {
-        const double m = mass ? mass[fi] : 1.0;
-        double v[3] = {0, 0, 0};
-        if (vel) { v[0] = vel[fi * 3]; v[1] = vel[fi * 3 + 1]; v[2] = vel[fi * 3 + 2]; }
-        double *kk = skin + fk * KN;
-        // symmetric packing: diagonal, then (0,1) [, (0,2), (1,2)]
-        for (int a = 0; a < OD; ++a) kk[a] = m * v[a] * v[a];
-        if (OD == 2) kk[2] = m * v[0] * v[1];
-        else { kk[3] = m * v[0] * v[1]; kk[4] = m * v[0] * v[2]; kk[5] = m * v[1] * v[2]; }
-    }
-    if (sdisp)
-        for (int a = 0; a < 3; ++a) sdisp[fk * 3 + a] = disp ? disp[fi * 3 + a] : 0.0;
+    const size_t fo = (size_t)f * n;
+    sort_finish_body<D, OD>(g, n, bx * blockDim.x + threadIdx.x, ncell, keys + fo, vals + fo, pos + fo * 3,
+                            diam + fo, mass ? mass + fo : nullptr, vel ? vel + fo * 3 : nullptr,
+                            disp ? disp + fo * 3 : nullptr, cstart + (size_t)f * (ncell + 1), spos + fo * PS,
+                            sdiam + fo, skin ? skin + fo * KN : nullptr, sdisp ? sdisp + fo * 3 : nullptr,
+                            rec ? rec + fo : nullptr);
 }
 
 // Clears that a captured HIP graph can hold (a memset node is avoided there).

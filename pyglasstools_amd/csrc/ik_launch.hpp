@@ -5,6 +5,7 @@
 
 #include "ik_device.hpp"
 #include "ik_geom.hpp"
+#include "ik_sweep.hpp"
 
 namespace pgt {
 
@@ -77,5 +78,23 @@ inline int adaptive_lanes(int64_t G)
 void launch_gather_adaptive(const LaunchCtx &c, unsigned fields, int cg, int D, int OD, AdaptQ Q, int *capped,
                             double *ws, int wcap);
 size_t adaptive_workspace_doubles(const LaunchCtx &c, int wcap);  // ik_gather_adaptive.hip
+
+// The field gather of a sweep (ik_sweep.hpp): one launch over the gather
+// ranges of every system, `blocks` workgroups of kTileThreads lanes.
+struct SweepGatherCtx {
+    const SweepSys *tab;  // device copy of the sweep's table
+    int ns;
+    int blocks;
+    GLRule R;
+    const int *cstart;
+    const double *spos, *skin;
+    int cap;
+    const int *bcount;
+    const void *bonds;
+    unsigned long long *wave_counts;
+    int *flags;
+    hipStream_t stream;
+};
+void launch_sweep_gather(const SweepGatherCtx &c, bool fp32, unsigned fields);  // ik_sweep_gather.hip
 
 }  // namespace pgt

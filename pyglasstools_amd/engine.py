@@ -28,6 +28,8 @@ EXPORTED = [
     "pgt_ik_set_tuning", "pgt_ik_set_gather_mode", "pgt_ik_get_stats", "pgt_eval_pair_force", "pgt_eval_cgfunc",
     "pgt_mlp_nparam", "pgt_mlp_work_floats", "pgt_mlp_forward", "pgt_mlp_grad", "pgt_mlp_adam", "pgt_mlp_infer",
     "pgt_ik_bond_samples", "pgt_ik_compute_surrogate", "pgt_pretest_band",
+    "pgt_ik_sweep_create", "pgt_ik_sweep_destroy", "pgt_ik_sweep_run", "pgt_ik_sweep_enqueue",
+    "pgt_ik_sweep_release", "pgt_ik_sweep_finish", "pgt_ik_sweep_set_stream", "pgt_ik_sweep_get_stats",
 ]
 
 
@@ -47,6 +49,20 @@ class Params(ctypes.Structure):
         ("adapt_epsabs", ctypes.c_double),
         ("precision", ctypes.c_int),
         ("max_rcut", ctypes.c_double),
+    ]
+
+
+class SweepSystem(ctypes.Structure):
+    _fields_ = [
+        ("engine", ctypes.c_void_p),
+        ("nframes", ctypes.c_int32),
+        ("n", ctypes.c_int64),
+        ("d_pos", ctypes.c_void_p),
+        ("d_diam", ctypes.c_void_p),
+        ("d_mass", ctypes.c_void_p),
+        ("d_vel", ctypes.c_void_p),
+        ("d_out_virial", ctypes.c_void_p),
+        ("d_out_kinetic", ctypes.c_void_p),
     ]
 
 
@@ -109,8 +125,14 @@ def lib():
         L.pgt_mlp_infer.argtypes = [vp, i64, vp, vp, vp]
         L.pgt_ik_bond_samples.argtypes = [vp, i64, vp, vp, ctypes.POINTER(i64)]
         L.pgt_ik_compute_surrogate.argtypes = [vp, vp, d, d, d, vp]
+        L.pgt_ik_sweep_create.argtypes = [i, ctypes.POINTER(SweepSystem), ctypes.c_uint, vp, ctypes.POINTER(vp)]
+        L.pgt_ik_sweep_destroy.argtypes = [vp]
+        for name in ("pgt_ik_sweep_run", "pgt_ik_sweep_enqueue", "pgt_ik_sweep_release", "pgt_ik_sweep_finish"):
+            getattr(L, name).argtypes = [vp]
+        L.pgt_ik_sweep_set_stream.argtypes = [vp, vp]
+        L.pgt_ik_sweep_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         for name in EXPORTED:
-            if name not in ("pgt_last_error", "pgt_device_count", "pgt_ik_destroy"):
+            if name not in ("pgt_last_error", "pgt_device_count", "pgt_ik_destroy", "pgt_ik_sweep_destroy"):
                 getattr(L, name).restype = i
         L.pgt_mlp_work_floats.restype = ctypes.c_size_t
         _LIB = L
@@ -373,6 +395,82 @@ class Engine(object):
     def stats(self):
         s = Stats()
         _check(lib().pgt_ik_get_stats(self.h, ctypes.byref(s)), "pgt_ik_get_stats")
+        return s.as_dict()
+
+
+class Sweep(object):
+    """One pgt_ik_sweep: the frames of several 2D systems, one launch per
+    stage over all of them.  systems: dicts with the system's Engine (box,
+    grid and parameters set) under "engine", "nframes", "n", the device
+    inputs "pos", "diam" (and optionally "mass", "vel"; torch float64
+    tensors, frames back to back) and the outputs "virial" / "kinetic"
+    (nframes x G x 2 x 2) for the selected fields.  The sweep keeps the
+    tensors alive and reads and writes them on every run."""
+
+    def __init__(self, systems, fields=("virial", "kinetic"), stream_handle=None):
+        import torch
+
+        mask = (FIELD_VIRIAL if "virial" in fields else 0) | (FIELD_KINETIC if "kinetic" in fields else 0) | \
+               (FIELD_ELASTIC if "elastic" in fields else 0)
+        f64 = torch.float64
+        arr = (SweepSystem * len(systems))()
+        self._keep = []
+        for k, y in enumerate(systems):
+            e, nf, n = y["engine"], int(y["nframes"]), int(y["n"])
+            if nf < 1 or n < 1:
+                raise ValueError("system %d: frame and particle counts must be positive" % k)
+            m, fsz = nf * n, nf * e.G * 4
+            if e.dim != 2:
+                raise ValueError("system %d: a sweep runs 2D engines" % k)
+            a = arr[k]
+            a.engine = e.h.value if isinstance(e.h, ctypes.c_void_p) else e.h
+            a.nframes, a.n = nf, n
+            a.d_pos = _dev(y["pos"], f64, 3 * m, "pos").value
+            a.d_diam = _dev(y["diam"], f64, m, "diam").value
+            p = _dev(y.get("mass"), f64, m, "mass")
+            a.d_mass = p.value if p is not None else None
+            p = _dev(y.get("vel"), f64, 3 * m, "vel")
+            a.d_vel = p.value if p is not None else None
+            for name, bit in (("virial", FIELD_VIRIAL), ("kinetic", FIELD_KINETIC)):
+                if mask & bit:
+                    if y.get(name) is None:
+                        raise ValueError("system %d: %s output is required for the selected fields" % (k, name))
+                    setattr(a, "d_out_" + name, _dev(y[name], f64, fsz, name).value)
+            self._keep.append(y)
+        self.systems = list(systems)
+        h = ctypes.c_void_p()
+        _check(lib().pgt_ik_sweep_create(len(systems), arr, mask,
+                                         ctypes.c_void_p(stream_handle) if stream_handle else None, ctypes.byref(h)),
+               "pgt_ik_sweep_create")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _LIB is not None:
+            _LIB.pgt_ik_sweep_destroy(h)
+            self.h = None
+
+    def run(self):
+        _check(lib().pgt_ik_sweep_run(self.h), "pgt_ik_sweep_run")
+
+    def enqueue(self):
+        """The sweep's launches, no synchronisation or allocation (graph
+        capture); needs an earlier run(); check with finish()."""
+        _check(lib().pgt_ik_sweep_enqueue(self.h), "pgt_ik_sweep_enqueue")
+
+    def release(self):
+        _check(lib().pgt_ik_sweep_release(self.h), "pgt_ik_sweep_release")
+
+    def finish(self):
+        _check(lib().pgt_ik_sweep_finish(self.h), "pgt_ik_sweep_finish")
+
+    def set_stream(self, stream_handle):
+        _check(lib().pgt_ik_sweep_set_stream(self.h, ctypes.c_void_p(stream_handle) if stream_handle else None),
+               "pgt_ik_sweep_set_stream")
+
+    def stats(self):
+        s = Stats()
+        _check(lib().pgt_ik_sweep_get_stats(self.h, ctypes.byref(s)), "pgt_ik_sweep_get_stats")
         return s.as_dict()
 
 
