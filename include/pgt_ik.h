@@ -227,10 +227,10 @@ int pgt_ik_set_stream(pgt_ik_engine *e, void *hip_stream);
  * (pyglasstools/__init__.py:119-122) computes one system's frames one call
  * at a time.  A sweep computes the frames of several 2D systems (each its own
  * engine: box, grid, potential, kernel radius, cell reach; particle counts may
- * differ) with one launch per stage over all of them: cell-list passes, sorted
- * arrays, bond list, field gather, counters (3 x passes + 4 launches, 7 or 10
- * in all), so a captured graph holds about ten nodes whatever the number of
- * systems.  Each system's fields equal its engine's pgt_ik_compute_frames
+ * differ) with one launch per stage over all of them: cell-list passes (3
+ * launches per radix pass of the cell keys, 1 to 3 passes), sorted arrays,
+ * bond list, field gather, counters, so a captured graph holds 7 to 13
+ * nodes whatever the number of systems.  Each system's fields equal its engine's pgt_ik_compute_frames
  * with the tile gather, bit for bit.  Forms: 2D (positions and grid with
  * z = 0), Octic kernel, fixed 5-node rule, FP64 or FP32 bond entries (the
  * same for every system), virial and/or kinetic fields; others give
