@@ -513,9 +513,15 @@ __global__ __launch_bounds__(64, (TileWaves<D, ELA>::value)) void k_gather_tile(
 // The sweep's field gather (2D, virial and/or kinetic): each workgroup finds
 // its system, frame and tile in the sweep's table (ik_sweep.hpp).  Padding
 // workgroups store zero counts for their slot, so k_reduce_stats can read
-// every slot.
+// every slot.  Three waves per SIMD: the system's geometry and pointers come
+// from the table and stay in registers (a kernel argument would be reloaded
+// instead), which at the single-system kernel's four waves spills 68 bytes a
+// lane in FP64; with the room of three there is no spill, and the eight-system
+// sweep's gather takes 6.27 ms instead of 6.54.
+constexpr int kSweepGatherWaves = 3;
+
 template <bool VIR, bool KIN, typename ER>
-__global__ __launch_bounds__(64, (TileWaves<2, false>::value)) void k_sweep_gather(
+__global__ __launch_bounds__(64, kSweepGatherWaves) void k_sweep_gather(
     const SweepSys *__restrict__ T, int ns, GLRule R, const int *__restrict__ cstart,
     const double *__restrict__ spos, const double *__restrict__ skin, int cap, const int *__restrict__ bcount,
     const ER *__restrict__ bonds, unsigned long long *__restrict__ wave_counts, int *__restrict__ overflow)
