@@ -1580,13 +1580,13 @@ int pgt_mlp_forward(const float *d_params, int64_t n, const float *d_x, float *d
     hipStream_t s = (hipStream_t)stream;
     const float *P = d_params;
     float *H1 = d_work, *H2 = d_work + 64 * n, *H3 = d_work + 128 * n;
-    hipLaunchKernelGGL(k_dense_fwd, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpIn, kMlpW, d_x, kMlpInPad,
+    hipLaunchKernelGGL((k_dense_fwd<kMlpIn, kMlpW>), dim3(blocks_for(n * 64)), dim3(256), 0, s, n, d_x, kMlpInPad,
                        P + kOffW1, P + kOffB1, true, H1);
-    hipLaunchKernelGGL(k_dense_fwd, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpW, kMlpW, H1, kMlpW,
+    hipLaunchKernelGGL((k_dense_fwd<kMlpW, kMlpW>), dim3(blocks_for(n * 64)), dim3(256), 0, s, n, H1, kMlpW,
                        P + kOffW2, P + kOffB2, true, H2);
-    hipLaunchKernelGGL(k_dense_fwd, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpW, kMlpW, H2, kMlpW,
+    hipLaunchKernelGGL((k_dense_fwd<kMlpW, kMlpW>), dim3(blocks_for(n * 64)), dim3(256), 0, s, n, H2, kMlpW,
                        P + kOffW3, P + kOffB3, true, H3);
-    hipLaunchKernelGGL(k_dense_fwd, dim3(blocks_for(n)), dim3(256), 0, s, n, kMlpW, 1, H3, kMlpW, P + kOffW4,
+    hipLaunchKernelGGL((k_dense_fwd<kMlpW, 1>), dim3(blocks_for(n)), dim3(256), 0, s, n, H3, kMlpW, P + kOffW4,
                        P + kOffB4, false, d_y);
     HIPCHK(hipGetLastError());
     return PGT_OK;
@@ -1612,19 +1612,17 @@ int pgt_mlp_grad(const float *d_params, int64_t n, const float *d_x, const float
     // weight and bias gradient of a layer with K inputs X (row stride ldx), N outputs, error dZ
     auto wgrad = [&](int K, int N, const float *X, int ldx, const float *dZ, float *gW, float *gb) {
         const int M = K * N + N;
-        hipLaunchKernelGGL(k_dense_bwd_weight, dim3(blocks_for(M), kGradSplit), dim3(256), 0, s, n, K, N, X, ldx, dZ,
-                           gpart);
-        hipLaunchKernelGGL(k_sum_split, dim3(blocks_for(M)), dim3(256), 0, s, kGradSplit, K, N, gpart, gW, gb);
+        hipLaunchKernelGGL(k_dense_bwd_weight, dim3(kGradSplit), dim3(256), 0, s, n, K, N, X, ldx, dZ, gpart);
+        hipLaunchKernelGGL(k_sum_split, dim3(blocks_for(M)), dim3(256), 0, s, K, N, gpart, gW, gb);
     };
     wgrad(kMlpW, 1, H3, kMlpW, dY, g + kOffW4, g + kOffB4);
-    hipLaunchKernelGGL(k_dense_bwd_data, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpW, 1, dY, P + kOffW4, H3,
-                       dZa);
+    hipLaunchKernelGGL(k_dense_bwd_data<1>, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, dY, P + kOffW4, H3, dZa);
     wgrad(kMlpW, kMlpW, H2, kMlpW, dZa, g + kOffW3, g + kOffB3);
-    hipLaunchKernelGGL(k_dense_bwd_data, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpW, kMlpW, dZa, P + kOffW3,
-                       H2, dZb);
+    hipLaunchKernelGGL(k_dense_bwd_data<kMlpW>, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, dZa, P + kOffW3, H2,
+                       dZb);
     wgrad(kMlpW, kMlpW, H1, kMlpW, dZb, g + kOffW2, g + kOffB2);
-    hipLaunchKernelGGL(k_dense_bwd_data, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, kMlpW, kMlpW, dZb, P + kOffW2,
-                       H1, dZa);
+    hipLaunchKernelGGL(k_dense_bwd_data<kMlpW>, dim3(blocks_for(n * 64)), dim3(256), 0, s, n, dZb, P + kOffW2, H1,
+                       dZa);
     wgrad(kMlpIn, kMlpW, d_x, kMlpInPad, dZa, g + kOffW1, g + kOffB1);
     HIPCHK(hipGetLastError());
     return PGT_OK;

@@ -33,17 +33,24 @@ constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kMlpIn * kMlpW, kOffW2 = kOffB1 + kM
 // FP32 training kernels (row-major activations, one sample per row)
 // ------------------------------------------------------------------------
 
-// Y[s][o] = act(b[o] + sum_k X[s][k] W[k][o]); X rows have stride ldx.
-__global__ void k_dense_fwd(int64_t n, int K, int N, const float *__restrict__ X, int ldx,
-                            const float *__restrict__ W, const float *__restrict__ b, bool tanh_act,
-                            float *__restrict__ Y)
+// Y[s][o] = act(b[o] + sum_k X[s][k] W[k][o]); X rows have stride ldx.  The
+// layer sizes are template arguments, so a thread's K loads of X and of W
+// issue together instead of one per step of a loop of unknown length.
+template <int K, int N>
+__global__ __launch_bounds__(256) void k_dense_fwd(int64_t n, const float *__restrict__ X, int ldx,
+                                                   const float *__restrict__ W, const float *__restrict__ b,
+                                                   bool tanh_act, float *__restrict__ Y)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * N) return;
     const int64_t s = idx / N;
     const int o = (int)(idx % N);
+    float x[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) x[k] = X[s * ldx + k];
     float acc = b[o];
-    for (int k = 0; k < K; ++k) acc = fmaf(X[s * ldx + k], W[k * N + o], acc);
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc = fmaf(x[k], W[k * N + o], acc);
     Y[idx] = tanh_act ? tanhf(acc) : acc;
 }
 
@@ -86,52 +93,104 @@ __global__ __launch_bounds__(256) void k_sum_parts(int np, const double *__restr
 }
 
 // dZp[s][k] = (sum_o dZ[s][o] W[k][o]) * (1 - H[s][k]^2): the error at the
-// pre-activation of the previous (tanh) layer, H being that layer's output.
-__global__ void k_dense_bwd_data(int64_t n, int K, int N, const float *__restrict__ dZ, const float *__restrict__ W,
-                                 const float *__restrict__ H, float *__restrict__ dZp)
+// pre-activation of the previous (tanh) layer of K = kMlpW units, H being
+// that layer's output; N outputs (a template argument, for the same reason
+// as in k_dense_fwd).
+template <int N>
+__global__ __launch_bounds__(256) void k_dense_bwd_data(int64_t n, const float *__restrict__ dZ,
+                                                        const float *__restrict__ W, const float *__restrict__ H,
+                                                        float *__restrict__ dZp)
 {
+    constexpr int K = kMlpW;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * K) return;
     const int64_t s = idx / K;
     const int k = (int)(idx % K);
+    float z[N];
+#pragma unroll
+    for (int o = 0; o < N; ++o) z[o] = dZ[s * N + o];
     float acc = 0.0f;
-    for (int o = 0; o < N; ++o) acc = fmaf(dZ[s * N + o], W[k * N + o], acc);
+#pragma unroll
+    for (int o = 0; o < N; ++o) acc = fmaf(z[o], W[k * N + o], acc);
     const float h = H[idx];
     dZp[idx] = acc * (1.0f - h * h);
 }
 
 // Weight and bias gradients in two deterministic stages.  Stage 1: workgroup
-// row c (blockIdx.y, of gridDim.y) sums samples [n c / S, n (c+1) / S) into
-// part[c][k*N + o] = sum X[s][k] dZ[s][o] and part[c][K*N + o] = sum dZ[s][o].
-// Stage 2 (k_sum_split) adds the S rows in order.
+// c (of S = gridDim.x) sums samples [n c / S, n (c+1) / S) into
+// part[c][k*N + o] = sum X[s][k] dZ[s][o] and part[c][K*N + o] = sum dZ[s][o]
+// (K, N <= 64).  The slice is staged through LDS kWChunk samples at a time
+// (the X and dZ rows of the chunk), and thread t keeps the 4 x 4 outputs
+// k = 4 (t / 16) + i, o = 4 (t % 16) + j in registers, adding the chunk's
+// samples in order: 16 FMAs per pair of 16-byte LDS reads, against a
+// dependent chain of global loads per output before.  Stage 2 (k_sum_split)
+// adds the S rows in order.
 constexpr int kGradSplit = 64;
-__global__ void k_dense_bwd_weight(int64_t n, int K, int N, const float *__restrict__ X, int ldx,
-                                   const float *__restrict__ dZ, float *__restrict__ part)
+constexpr int kWChunk = 32;
+__global__ __launch_bounds__(256) void k_dense_bwd_weight(int64_t n, int K, int N, const float *__restrict__ X,
+                                                          int ldx, const float *__restrict__ dZ,
+                                                          float *__restrict__ part)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int M = K * N + N;
-    if (t >= M) return;
-    const int c = blockIdx.y, S = gridDim.y;
+    __shared__ float4 sx[kWChunk][kMlpW / 4];  // X rows of the chunk, zero past K
+    __shared__ float4 sz[kWChunk][kMlpW / 4];  // dZ rows, zero past N
+    const int t = threadIdx.x;
+    const int ty = t >> 4, tx = t & 15;
+    const int c = blockIdx.x, S = gridDim.x;
     const int64_t s0 = n * c / S, s1 = n * (c + 1) / S;
-    float acc = 0.0f;
-    if (t < K * N) {
-        const int k = t / N, o = t % N;
-        for (int64_t s = s0; s < s1; ++s) acc = fmaf(X[s * ldx + k], dZ[s * N + o], acc);
-    } else {
-        const int o = t - K * N;
-        for (int64_t s = s0; s < s1; ++s) acc += dZ[s * N + o];
+    float acc[4][4], bacc = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    float *fx = reinterpret_cast<float *>(sx), *fz = reinterpret_cast<float *>(sz);
+    for (int64_t b = s0; b < s1; b += kWChunk) {
+        __syncthreads();  // the previous chunk's reads are done
+        for (int q = t; q < kWChunk * kMlpW; q += 256) {
+            const int r = q / kMlpW, k = q % kMlpW;
+            const int64_t smp = b + r;
+            const bool ok = smp < s1;
+            fx[q] = (ok && k < K) ? X[smp * ldx + k] : 0.0f;
+            fz[q] = (ok && k < N) ? dZ[smp * N + k] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int r = 0; r < kWChunk; ++r) {
+            const float4 a = sx[r][ty], z = sz[r][tx];
+            const float av[4] = {a.x, a.y, a.z, a.w}, zv[4] = {z.x, z.y, z.z, z.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(av[i], zv[j], acc[i][j]);
+        }
+        if (t < N)
+            for (int r = 0; r < kWChunk; ++r) bacc += fz[r * kMlpW + t];
     }
-    part[(size_t)c * M + t] = acc;
+    const int M = K * N + N;
+    float *out = part + (size_t)c * M;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = 4 * ty + i, o = 4 * tx + j;
+            if (k < K && o < N) out[k * N + o] = acc[i][j];
+        }
+    if (t < N) out[K * N + t] = bacc;
 }
 
-__global__ void k_sum_split(int S, int K, int N, const float *__restrict__ part, float *__restrict__ gW,
-                            float *__restrict__ gb)
+// gW, gb = the sum of the kGradSplit rows of part, in row order (the rows'
+// loads issue together).
+__global__ __launch_bounds__(256) void k_sum_split(int K, int N, const float *__restrict__ part,
+                                                   float *__restrict__ gW, float *__restrict__ gb)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int M = K * N + N;
     if (t >= M) return;
+    float v[kGradSplit];
+#pragma unroll
+    for (int c = 0; c < kGradSplit; ++c) v[c] = part[(size_t)c * M + t];
     float acc = 0.0f;
-    for (int c = 0; c < S; ++c) acc += part[(size_t)c * M + t];
+#pragma unroll
+    for (int c = 0; c < kGradSplit; ++c) acc += v[c];
     if (t < K * N) gW[t] = acc;
     else gb[t - K * N] = acc;
 }
