@@ -142,6 +142,55 @@ def test_sweep_graph_replays_repeat_the_run(gpu):
     sw.release()
 
 
+def test_sweep_mixed_geometries_match_engines_and_oracle(gpu):
+    """Systems whose cell geometry differs in kind, in one sweep: a box under
+    three cell widths (one cell, minimum image; r_cg 1.0), a box open along y
+    (one cell along y, no minimum image there), and a periodic one at cell
+    reach 2; each against its engine bit for bit and against the oracle."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    specs = [(30, 2, 6, (1, 1, 0), 1.0, 1), (800, 3, 16, (1, 0, 0), 2.0, 1), (1500, 2, 20, (1, 1, 0), 2.0, 2)]
+    systems = []
+    for k, (N, FR, NG, per, rcg, reach) in enumerate(specs):
+        frames = [workload.make_system(N, 2, 9000 + 10 * k + f) for f in range(FR)]
+        for f in frames:
+            f["periodic"] = np.array(per)
+        L = frames[0]["L"]
+        d0 = max(float(f["diam"].max()) for f in frames)
+        max_rcut = max(1.25 * d0, d0)
+        grid = workload.cartesian_grid_2d(L[0], NG)
+        e = engine.Engine(dim=2, max_rcut=max_rcut, cg_rcut=rcg)
+        e.set_box(L, list(per))
+        e.set_grid(grid)
+        e.set_gather_mode("tile")
+        e.set_tuning(cell_reach=reach)
+        G = len(grid)
+        y = dict(engine=e, nframes=FR, n=N, frames=frames, grid=grid, max_rcut=max_rcut, cg_rcut=rcg,
+                 pos=_stack(frames, "pos", dev), diam=_stack(frames, "diam", dev),
+                 mass=_stack(frames, "mass", dev), vel=_stack(frames, "vel", dev))
+        for name in ("virial", "kinetic", "ref_virial", "ref_kinetic"):
+            y[name] = torch.full((FR, G, 2, 2), float("nan"), dtype=torch.float64, device=dev)
+        systems.append(y)
+    fields = ("virial", "kinetic")
+    _engine_reference(systems, fields)
+    sw = engine.Sweep(systems, fields=fields)
+    sw.run()
+    _assert_same(systems, fields, "mixed geometries")
+    n_nonzero = 0
+    for k, y in enumerate(systems):
+        for f in range(y["nframes"]):
+            ref = oracle.ik_compute(y["frames"][f], y["grid"], fields=fields, max_rcut=y["max_rcut"],
+                                    cg_rcut=y["cg_rcut"], with_abs=True)
+            n_nonzero += ref["n_nonzero"]
+            err = scaled_error(y["virial"][f].cpu().numpy(), ref["virial"], ref["virial_abs"], floor=1e-4).max()
+            print("mixed geometries: system %d frame %d scaled err %.2e" % (k, f, err), flush=True)
+            assert err <= TOL, (k, f, err)
+            gk = y["kinetic"][f].cpu().numpy()
+            assert np.abs(gk - ref["kinetic"]).max() <= 1e-12 * np.abs(ref["kinetic"]).max(), (k, f)
+    assert sw.stats()["n_contrib"] == n_nonzero
+
+
 def test_sweep_grows_bond_capacity(gpu):
     """A pair potential cut off at 2.8 sigma gives about 22 bonds a
     particle, past the initial capacity of 16 of engines that have not run:
