@@ -51,10 +51,12 @@ def test_mlp_forward_fp32_matches_oracle(gpu):
     assert err <= 2e-5 * max(1.0, np.abs(ref).max())
 
 
-def test_mlp_gradient_matches_oracle(gpu):
+@pytest.mark.parametrize("n", [257, 5003])
+def test_mlp_gradient_matches_oracle(gpu, n):
+    """n = 257: one partial 32-sample chunk per weight-gradient workgroup;
+    n = 5003: two or three chunks each, the last one ragged."""
     import torch
 
-    n = 257
     p, x = _params(5), _rows(n, 6)
     t = np.random.default_rng(8).uniform(0.0, 0.6, n).astype(np.float32)
     dev = torch.device("cuda:0")
