@@ -16,7 +16,7 @@ constexpr int kSweepMaxSystems = 16;
 
 // The stages with per-system workgroup ranges: radix-sort histogram and
 // scatter (one chunk of kSortChunk particles per workgroup), the sorted
-// arrays (256 particles), the bond list (kBondBlock particles) and the field
+// arrays (256 particles), the bond list (256 particles) and the field
 // gather (kTilePoints grid points).
 enum SweepStage { SW_SORT = 0, SW_FINISH = 1, SW_BONDS = 2, SW_GATHER = 3, SW_NSTAGE = 4 };
 
