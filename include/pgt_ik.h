@@ -261,7 +261,10 @@ void pgt_ik_sweep_destroy(pgt_ik_sweep *s);
 
 /* Run the sweep and wait for it: sizes the work arrays and grows the bond
  * capacity (running again) as pgt_ik_compute_frames does; sets the counts of
- * pgt_ik_sweep_get_stats (totals over every system). */
+ * pgt_ik_sweep_get_stats, totals over every frame of every system:
+ * n_particles and n_grid count particles and grid points summed over frames,
+ * gather_ms and total_ms are the gather's and the whole sweep's HIP-event
+ * times. */
 int pgt_ik_sweep_run(pgt_ik_sweep *s);
 
 /* Enqueue the sweep's launches and return: no synchronisation, allocation,
