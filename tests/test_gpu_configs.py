@@ -12,7 +12,8 @@ through the C ABI:
     its contribution count within 1e-6 of it (pairs at the cutoff);
   8 systems of uneven N (4k .. 96k), 128 x 128 grid each, two frames per
     system, captured into one HIP graph on streams forked inside it and
-    replayed: every system's frames against the oracle, FP64 and FP32.
+    replayed, and the same systems through one sweep (one launch per stage):
+    every system's frames against the oracle, FP64 and FP32.
 """
 import numpy as np
 import pytest
@@ -228,24 +229,42 @@ def test_c5_graph_sweep_matches_oracle(gpu, precision):
         torch.cuda.synchronize()
         for j in jobs:
             j["e"].finish()
+    def check(j, f, ref, gv, gk, what):
+        ek = np.abs(gk - ref["kinetic"]).max() / np.abs(ref["kinetic"]).max()
+        if precision == "fp64":
+            ev = assert_fp64_parity(gv, ref["virial"], ref["virial_abs"], "c5 %s N=%d frame %d" % (what, j["N"], f))
+            ok = True
+        else:
+            ev = _fp32_rel(gv, ref["virial"])
+            ok = ev <= TOL_FP32
+        print("c5 %s %s N=%d frame %d: virial err %.3e, kinetic %.3e" % (what, precision, j["N"], f, ev, ek),
+              flush=True)
+        assert ok and ek <= TOL
+
+    refs = {}
+    total = 0
     for j in jobs:
         st = j["e"].stats()  # counts of the last replay
         n_nonzero = n_ball = 0
         for f in range(FR):
             ref = oracle.ik_compute(j["frames"][f], j["grid"], fields=("virial", "kinetic"), max_rcut=j["max_rcut"],
                                     with_abs=True)
+            refs[(j["N"], f)] = ref
             n_nonzero += ref["n_nonzero"]
             n_ball += ref["n_ball"]
-            gv, gk = j["ov"][f].cpu().numpy(), j["ok"][f].cpu().numpy()
-            ek = np.abs(gk - ref["kinetic"]).max() / np.abs(ref["kinetic"]).max()
-            if precision == "fp64":
-                ev = assert_fp64_parity(gv, ref["virial"], ref["virial_abs"], "c5 N=%d frame %d" % (j["N"], f))
-                ok = True
-            else:
-                ev = _fp32_rel(gv, ref["virial"])
-                ok = ev <= TOL_FP32
-            print("c5 %s N=%d frame %d: virial err %.3e, kinetic %.3e" % (precision, j["N"], f, ev, ek), flush=True)
-            assert ok and ek <= TOL
+            check(j, f, ref, j["ov"][f].cpu().numpy(), j["ok"][f].cpu().numpy(), "graph")
         assert st["n_contrib"] == n_nonzero and st["n_ball"] == n_ball
+        total += n_nonzero
         j["e"].set_stream(0)
         j["e"].release_enqueued()
+    # the same 8 systems through one sweep (one launch per stage, tile gather)
+    outs = [(torch.full_like(j["ov"], float("nan")), torch.full_like(j["ok"], float("nan"))) for j in jobs]
+    sw = engine.Sweep([dict(engine=j["e"], nframes=FR, n=j["N"], pos=j["pos"], diam=j["diam"], mass=j["mass"],
+                            vel=j["vel"], virial=o[0], kinetic=o[1]) for j, o in zip(jobs, outs)],
+                      fields=("virial", "kinetic"))
+    print("c5 %s: sweep" % precision, flush=True)
+    sw.run()
+    for j, o in zip(jobs, outs):
+        for f in range(FR):
+            check(j, f, refs[(j["N"], f)], o[0][f].cpu().numpy(), o[1][f].cpu().numpy(), "sweep")
+    assert sw.stats()["n_contrib"] == total
