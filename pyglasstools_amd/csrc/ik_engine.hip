@@ -441,19 +441,23 @@ Geom make_geom(const pgt_ik_engine *e, int D)
 // ------------------------------------------------------------------------
 // launch dispatch
 // ------------------------------------------------------------------------
-// k_bonds threads per workgroup: 256 in 2D (a 32-entry candidate list per
-// lane, 32 KB of LDS; 1% faster per step than 128 on the 2D N = 65,536 job),
-// 128 in 3D (a 64-entry list, where 256 lanes would need 64 KB)
-constexpr int bond_block(int D) { return D == 2 ? 256 : 128; }
+// k_bonds threads per workgroup and LDS candidate-list length per lane (a
+// full list is processed early: the length must stay well above the ~11.5
+// (2D) or ~33 (3D) neighbours within max_rcut at unit density, or most
+// particles process their list twice).  Measured per step: 2D, 256 lanes and
+// 32 entries (32 KB of LDS) 1% faster than 128 lanes on the N = 65,536 job,
+// 24 entries equal, 16 entries 17% slower; 3D, 64 lanes and 48 entries
+// (12 KB) 9% faster on the N = 131,072 FP32 job than 128 lanes and 64
+// entries, 40 entries 12% slower.
+constexpr int bond_block(int D) { return D == 2 ? 256 : 64; }
+constexpr int bond_list(int D) { return D == 2 ? 32 : 48; }
 
 template <int D, int OD, typename ER>
 void launch_bonds_t(const LaunchCtx &c, unsigned fields, const PotParams &P, const double *sdiam, const double *sdisp,
                     int *bcount, void *bonds, int *wave_max, float4 *rec)
 {
     const bool V = fields & PGT_FIELD_VIRIAL, E = fields & PGT_FIELD_ELASTIC;
-    // candidate list per lane: ~2.5x the mean number of neighbours within
-    // max_rcut at unit density (13 in 2D, 35 in 3D for max_rcut ~ 2)
-    constexpr int BLK = bond_block(D), MAXC = D == 2 ? 32 : 64;
+    constexpr int BLK = bond_block(D), MAXC = bond_list(D);
     dim3 grid((unsigned)((c.n + BLK - 1) / BLK), (unsigned)c.nf);
     if (V && E)
         hipLaunchKernelGGL((k_bonds<D, OD, true, true, ER, BLK, MAXC>), grid, dim3(BLK), 0, c.stream, c.g, c.n, P,
@@ -1848,10 +1852,10 @@ int sweep_launch(pgt_ik_sweep *s, bool timed)
     if (s->vir) {
         const dim3 bg((unsigned)s->blocks[SW_BONDS]);
         if (s->fp32)
-            hipLaunchKernelGGL((k_sweep_bonds<float, bond_block(2), 32>), bg, dim3(bond_block(2)), 0, st, T, ns, s->cap,
+            hipLaunchKernelGGL((k_sweep_bonds<float, bond_block(2), bond_list(2)>), bg, dim3(bond_block(2)), 0, st, T, ns, s->cap,
                                cstart, spos, sdiam, bcount, (float *)s->d_bonds.p, wmax);
         else
-            hipLaunchKernelGGL((k_sweep_bonds<double, bond_block(2), 32>), bg, dim3(bond_block(2)), 0, st, T, ns, s->cap,
+            hipLaunchKernelGGL((k_sweep_bonds<double, bond_block(2), bond_list(2)>), bg, dim3(bond_block(2)), 0, st, T, ns, s->cap,
                                cstart, spos, sdiam, bcount, (double *)s->d_bonds.p, wmax);
         HIPCHK(hipGetLastError());
     }
