@@ -58,11 +58,15 @@ __device__ __forceinline__ T wave_max(T v)
 }
 
 // Wavefronts per SIMD the compiler sizes registers for: four for the 2D forms
-// without elastic terms (a few registers spilled at most), which hides more
-// memory latency; the others are left to the compiler (two or three).
-template <int D, bool ELA>
+// without elastic terms (a few registers spilled at most) and for the 3D FP32
+// virial form (126 registers, no spill; 131 and three waves unbounded, with
+// the gather 7% slower on the 3D N = 131,072 job), which hides more memory
+// latency; the others are left to the compiler (two or three: the 3D FP64
+// virial form would spill 88 bytes a lane at four).
+template <int D, bool ELA, bool KIN, typename ER>
 struct TileWaves {
-    static constexpr int value = (D == 2 && !ELA) ? 4 : 1;
+    static constexpr bool F32 = std::is_same<ER, float>::value;
+    static constexpr int value = ((D == 2 && !ELA) || (D == 3 && F32 && !ELA && !KIN)) ? 4 : 1;
 };
 
 // The body works on one frame's arrays (cell starts st, sorted positions
@@ -488,7 +492,7 @@ __device__ __forceinline__ void gather_tile_body(const Geom &g, int n, int G, co
 // One frame per y row; block x takes the tile xcd_tile(x, gridDim.x), so
 // each XCD works through a contiguous eighth of the frame's points.
 template <int D, int OD, int CG, int NQ, bool VIR, bool ELA, bool KIN, typename ER>
-__global__ __launch_bounds__(64, (TileWaves<D, ELA>::value)) void k_gather_tile(
+__global__ __launch_bounds__(64, (TileWaves<D, ELA, KIN, ER>::value)) void k_gather_tile(
     Geom g, int n, int G, const double *__restrict__ grid, const int *__restrict__ perm, GLRule R,
     const double *__restrict__ gld, double cgnorm, const int *__restrict__ cstart, const double *__restrict__ spos,
     const double *__restrict__ skin, int cap, const int *__restrict__ bcount, const float4 *__restrict__ rec,
