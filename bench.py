@@ -43,12 +43,14 @@ def make_frames(nframes, n, config=2):
     return systems
 
 
-def traffic_from_profile(mode):
-    """HBM bytes per launch of the C2 field gather of the given kind ("tile"
-    or "lane") measured by rocprofv3 PMC counters (FETCH_SIZE and WRITE_SIZE,
-    calibrated on a device copy), if a committed summary exists
-    (profiles/gather_c2_<mode>_traffic.json)."""
-    p = os.path.join(ROOT, "profiles", "gather_c2_%s_traffic.json" % mode)
+def traffic_from_profile(mode, workload="c2"):
+    """HBM bytes per launch of the field gather of the given kind ("tile" or
+    "lane") on the given workload ("c2": 2D N = 65,536; "c3": 3D N = 131,072
+    FP32 form) measured by rocprofv3 PMC counters (FETCH_SIZE and
+    WRITE_SIZE, calibrated on a device copy), with the frames per launch they
+    were measured at, if a committed summary exists
+    (profiles/gather_<workload>_<mode>_traffic.json)."""
+    p = os.path.join(ROOT, "profiles", "gather_%s_%s_traffic.json" % (workload, mode))
     if os.path.exists(p):
         try:
             with open(p) as f:
@@ -273,6 +275,9 @@ def precision_form(args):
     avg_launch_s = gather_ms / 1e3 / args.steps
     flops = FLOP_U_3D * n_contrib / args.steps
     achieved = flops / avg_launch_s / 1e12
+    traffic, traffic_fpl = traffic_from_profile(tuned["gather"], "c3")
+    if traffic is not None and traffic_fpl != F:
+        traffic = traffic * F / traffic_fpl  # per-launch bytes scale with frames per launch
 
     # error vs the FP64 oracle on a sample of grid points of frame 0; the FP64
     # GPU form on the same points for comparison
@@ -301,7 +306,7 @@ def precision_form(args):
                                    "forces and kernel weights, FP64 minimum image and accumulation, virial stress",
                        "frames_per_step": F, "n_particles": N, "grid": "64^3", "parallelism": "single GPU"},
             "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                          "kernel": ("k_gather_tile" if tuned["gather"] == "tile" else "k_gather") +
                                    " (field gather, FP32 form)",
                          "compute": "FP32 vector ALU (published MI355X FP32 peak 157.3 TFLOP/s, which counts packed "
@@ -314,6 +319,13 @@ def precision_form(args):
                       "tuning": tuned,
                       "error_vs_fp64_oracle": {"grid_points_sampled": int(len(idx)), "frame": 0,
                                                "per_component": comps, "worst_fp32": worst}}}
+    if traffic is not None:
+        # HBM bytes of the gather (PMC counters) over its launch time, against
+        # 8 TB/s; algorithmic: positions and diameters in, the field out
+        rl = line["roofline"]
+        rl["hbm_gbs"] = round(traffic / avg_launch_s / 1e9, 1)
+        rl["hbm_frac"] = round(traffic / avg_launch_s / 1e9 / PEAK_HBM_GBS, 4)
+        rl["algorithmic_bytes_per_launch"] = F * (N * (24 + 8) + G * 72)
     print(json.dumps(line), flush=True)
 
 
