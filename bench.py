@@ -476,6 +476,19 @@ def gs_nodes(graph):
         return None
 
 
+def sweep_roofline(r):
+    """Roofline of the sweep's FP64 field gather (one launch over every
+    system's frames): canonical flops of its contributions and ball pairs over
+    its HIP-event time in an eager run."""
+    t = r["sweep_gather_ms_eager"] / 1e3
+    achieved = r["sweep_gather_flops"] / t / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+            "kernel": "k_sweep_gather (tile field gather over all systems' frames)",
+            "compute": "FP64 vector ALU (published MI355X FP64 peak 78.6 TFLOP/s)",
+            "avg_launch_ms": round(t * 1e3, 4), "flops_per_launch": r["sweep_gather_flops"]}
+
+
 def batched_sweep(args):
     """Serving-style sweep: 1000 2D frames from 8 systems of uneven N (125
     frames each, seeds 5000 + 1000 k + f), each on a 128 x 128 grid over its
@@ -640,6 +653,7 @@ def batched_sweep(args):
                          "frames_per_s_sweep_eager": nframes / t_sweep_eager,
                          "sweep_graph_nodes": nodes,
                          "sweep_gather_ms_eager": sweep_stats["gather_ms"],
+                         "sweep_gather_flops": FLOP_U_2D * sweep_stats["n_contrib"] + FLOP_KIN_2D * sweep_stats["n_ball"],
                          "sweep_total_ms_eager": sweep_stats["total_ms"],
                          "sweep_max_rel_diff_vs_engines": dmax,
                          "contributions_per_s_graph": nc / args.steps / t_graph,
@@ -660,7 +674,7 @@ def batched_sweep(args):
                                    "r_cg=2.0, GL5, virial + kinetic; one sweep (one launch per stage over all "
                                    "systems' frames) captured in one HIP graph" % (list(C5_N),),
                        "frames_per_step": FR * len(C5_N), "grid": "128x128", "parallelism": "single GPU"},
-            "roofline": None, "cpu_baseline": None,
+            "roofline": sweep_roofline(results["fp64"]), "cpu_baseline": None,
             "extra": {"fp64": results["fp64"], "fp32_form": results["fp32"],
                       "fp32_over_fp64_graph": results["fp32"]["frames_per_s_graph"] / results["fp64"]["frames_per_s_graph"]}}
     print(json.dumps(line), flush=True)
