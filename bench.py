@@ -614,6 +614,9 @@ def batched_sweep(args):
         # for all 1000 frames, the tile gather at each engine's tuned cell
         # reach; run once eagerly, then captured (about ten graph nodes)
         outs = [(torch.empty_like(j["ov"]), torch.empty_like(j["ok"])) for j in jobs]
+        if args.sweep_reach:
+            for j in jobs:
+                j["e"].set_tuning(cell_reach=args.sweep_reach)
         sw = engine.Sweep([dict(engine=j["e"], nframes=FR, n=j["sy"]["N"], pos=j["sy"]["pos"], diam=j["sy"]["diam"],
                                 mass=j["sy"]["mass"], vel=j["sy"]["vel"], virial=o[0], kinetic=o[1])
                            for j, o in zip(jobs, outs)], fields=("virial", "kinetic"), stream_handle=s.cuda_stream)
@@ -692,6 +695,8 @@ def main():
                     help="training epochs of the surrogate comparison in the config-2 line's extra (0: skip it)")
     ap.add_argument("--sweep-streams", type=int, default=4, choices=[1, 2, 4, 8],
                     help="streams the config-5 graph forks its systems onto")
+    ap.add_argument("--sweep-reach", type=int, default=0, choices=[0, 1, 2],
+                    help="cell reach of every system in the config-5 sweep (0: each engine's tuned reach)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
