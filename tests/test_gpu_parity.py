@@ -264,6 +264,30 @@ def test_3d_pairs_on_the_search_radius(gpu, precision):
                 assert np.abs(got["virial"][:, b, a] - r).max() <= 1e-5 * np.abs(r).max()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("gather", ["tile", "lane"])
+def test_3d_dense_neighbour_lists(gpu, precision, gather):
+    """A pair cutoff of 2 sigma puts ~140 candidates within max_rcut around
+    each 3D particle, past the bond kernel's 48-entry list (processed early
+    when fewer than a batch of slots are left), and ~33 bonds, past the
+    initial capacity of 32 bond slots: both the early processing and the
+    capacity growth run, against the oracle."""
+    sysd = workload.make_system(3000, 3, seed=81)
+    grid = workload.cartesian_grid_3d(sysd["L"][0], 8)
+    got, ref, st = run_both(sysd, grid, fields=("virial",), precision=precision, scaled_rcut=2.0, gather=gather)
+    print("3d dense: max bonds %d, capacity %d" % (st["max_bonds"], st["bond_capacity"]), flush=True)
+    assert st["bond_capacity"] >= st["max_bonds"] > 0
+    if precision == "fp64":
+        assert_parity(got, ref, st, ("virial",))
+    else:
+        assert st["n_ball"] == ref["n_ball"]
+        assert abs(st["n_contrib"] - ref["n_nonzero"]) <= 1e-6 * ref["n_nonzero"] + 2
+        for a in range(3):
+            for b in range(a, 3):
+                r = ref["virial"][:, b, a]
+                assert np.abs(got["virial"][:, b, a] - r).max() <= 1e-5 * np.abs(r).max()
+
+
 def test_2d_system_with_3d_fields_nonflat_z(gpu):
     """A 2D box whose particles sit off z=0 takes the general 3-component path."""
     sysd = workload.make_system(1500, 2, seed=5)
