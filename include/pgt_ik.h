@@ -225,15 +225,15 @@ int pgt_ik_set_stream(pgt_ik_engine *e, void *hip_stream);
 /* ---- sweep: many systems' frames, one launch per stage -------------------
  * No counterpart in the reference, whose analyze loop
  * (pyglasstools/__init__.py:119-122) computes one system's frames one call
- * at a time.  A sweep computes the frames of several 2D systems (each its own
- * engine: box, grid, potential, kernel radius, cell reach; particle counts may
- * differ) with one launch per stage over all of them: cell-list passes (3
+ * at a time.  A sweep computes the frames of several systems, all 2D or all
+ * 3D (each its own engine: box, grid, potential, kernel radius, cell reach;
+ * particle counts may differ) with one launch per stage over all of them: cell-list passes (3
  * launches per radix pass of the cell keys, 1 to 3 passes), sorted arrays,
  * bond list, field gather, counters, so a captured graph holds 7 to 13
  * nodes whatever the number of systems.  Each system's fields equal its engine's pgt_ik_compute_frames
  * with the tile gather, bit for bit.  Forms: 2D (positions and grid with
- * z = 0), Octic kernel, fixed 5-node rule, FP64 or FP32 bond entries (the
- * same for every system), virial and/or kinetic fields; others give
+ * z = 0) or 3D, Octic kernel, fixed 5-node rule, FP64 or FP32 bond entries
+ * (the same for every system), virial and/or kinetic fields; others give
  * PGT_ERR_UNSUPPORTED.  At create the sweep copies what it needs from the
  * engines (geometry, parameters, grid), so they may change or be destroyed
  * afterwards without affecting it; the device pointers in `systems` are kept
@@ -244,12 +244,12 @@ typedef struct pgt_ik_sweep_system {
     pgt_ik_engine *engine;       /* box, grid and parameters set */
     int32_t nframes;             /* >= 1 */
     int64_t n;                   /* particles per frame, 1 .. 2^26 */
-    const double *d_pos;         /* device, nframes * n * 3 (z = 0) */
+    const double *d_pos;         /* device, nframes * n * 3 (2D: z = 0) */
     const double *d_diam;        /* device, nframes * n */
     const double *d_mass;        /* device, nframes * n, or NULL (unit mass) */
     const double *d_vel;         /* device, nframes * n * 3, or NULL (at rest) */
-    double *d_out_virial;        /* device, nframes * G * 4 (fields with virial) */
-    double *d_out_kinetic;       /* device, nframes * G * 4 (fields with kinetic) */
+    double *d_out_virial;        /* device, nframes * G * D * D (fields with virial) */
+    double *d_out_kinetic;       /* device, nframes * G * D * D (fields with kinetic) */
 } pgt_ik_sweep_system;
 
 /* Create a sweep over nsys (1 .. 16) systems on the current device, launching

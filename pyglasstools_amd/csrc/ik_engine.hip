@@ -1756,6 +1756,7 @@ struct pgt_ik_sweep {
     hipStream_t stream = nullptr;
     unsigned fields = 0;
     bool vir = false, kin = false, fp32 = false;
+    int D = 2;             // 2: 2D systems (z = 0), 2 x 2 fields; 3: 3D, 3 x 3
     int ns = 0;
     std::vector<SweepSys> tab;
     GLRule R{};
@@ -1774,7 +1775,7 @@ struct pgt_ik_sweep {
     DBuf<double> d_grid;   // the systems' grids, back to back
     DBuf<int> d_perm;      // and their Morton orders
     DBuf<unsigned char> d_keys0, d_vals0, d_keys1, d_vals1, d_hist, d_cstart, d_spos, d_sdiam, d_bcount, d_skin;
-    DBuf<unsigned char> d_bonds, d_wcounts, d_wmax;
+    DBuf<unsigned char> d_bonds, d_wcounts, d_wmax, d_rec;
     DBuf<int> d_maxcount;                // [0] largest bond count, [1] tile overflow bits
     DBuf<unsigned long long> d_stats;    // balls, contributions
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -1800,16 +1801,18 @@ int fail_sweep_held()
 int sweep_buffers(pgt_ik_sweep *s)
 {
     const size_t np = (size_t)s->np;
-    const size_t want[13] = {np * sizeof(int), np * sizeof(int), np * sizeof(int), np * sizeof(int),
-                             (size_t)s->nh * sizeof(int), (size_t)s->ncs * sizeof(int), np * 2 * sizeof(double),
-                             np * sizeof(double), s->vir ? np * sizeof(int) : 0, s->kin ? np * 3 * sizeof(double) : 0,
+    const size_t PS = s->D == 2 ? 2 : 4, KN = s->D == 2 ? 3 : 6;
+    const size_t want[14] = {np * sizeof(int), np * sizeof(int), np * sizeof(int), np * sizeof(int),
+                             (size_t)s->nh * sizeof(int), (size_t)s->ncs * sizeof(int), np * PS * sizeof(double),
+                             np * sizeof(double), s->vir ? np * sizeof(int) : 0, s->kin ? np * KN * sizeof(double) : 0,
                              s->vir ? np * (size_t)s->cap * s->ebytes : 0,
                              (size_t)s->blocks[SW_GATHER] * 2 * sizeof(unsigned long long),
-                             s->vir ? (size_t)s->blocks[SW_BONDS] * (bond_block(2) / 64) * sizeof(int) : 0};
-    DBuf<unsigned char> *bufs[13] = {&s->d_keys0, &s->d_vals0, &s->d_keys1, &s->d_vals1, &s->d_hist,
+                             s->vir ? (size_t)s->blocks[SW_BONDS] * (bond_block(s->D) / 64) * sizeof(int) : 0,
+                             (s->vir && s->D == 3) ? np * sizeof(float4) : 0};
+    DBuf<unsigned char> *bufs[14] = {&s->d_keys0, &s->d_vals0, &s->d_keys1, &s->d_vals1, &s->d_hist,
                                      &s->d_cstart, &s->d_spos, &s->d_sdiam, &s->d_bcount, &s->d_skin,
-                                     &s->d_bonds, &s->d_wcounts, &s->d_wmax};
-    for (int k = 0; k < 13; ++k) {
+                                     &s->d_bonds, &s->d_wcounts, &s->d_wmax, &s->d_rec};
+    for (int k = 0; k < 14; ++k) {
         if (!want[k] || (bufs[k]->p && bufs[k]->n >= want[k])) continue;
         if (s->held) return fail_sweep_held();
         if (int rc = bufs[k]->ensure(want[k])) return rc;
@@ -1845,18 +1848,32 @@ int sweep_launch(pgt_ik_sweep *s, bool timed)
                                hist, kout, vout);
         }
     }
-    hipLaunchKernelGGL(k_sweep_finish, dim3((unsigned)s->blocks[SW_FINISH]), dim3(256), 0, st, T, ns,
-                       kb[s->npass & 1], vb[s->npass & 1], cstart, spos, sdiam, s->kin ? skin : nullptr);
+    float4 *rec = (s->vir && s->D == 3) ? (float4 *)s->d_rec.p : nullptr;
+    const dim3 fg((unsigned)s->blocks[SW_FINISH]);
+    const int *keys = kb[s->npass & 1], *vals = vb[s->npass & 1];
+    if (s->D == 2)
+        hipLaunchKernelGGL((k_sweep_finish<2, 2>), fg, dim3(256), 0, st, T, ns, keys, vals, cstart, spos, sdiam,
+                           s->kin ? skin : nullptr, (float4 *)nullptr);
+    else
+        hipLaunchKernelGGL((k_sweep_finish<3, 3>), fg, dim3(256), 0, st, T, ns, keys, vals, cstart, spos, sdiam,
+                           s->kin ? skin : nullptr, rec);
     HIPCHK(hipGetLastError());
     int *wmax = (int *)s->d_wmax.p;
     if (s->vir) {
         const dim3 bg((unsigned)s->blocks[SW_BONDS]);
-        if (s->fp32)
-            hipLaunchKernelGGL((k_sweep_bonds<float, bond_block(2), bond_list(2)>), bg, dim3(bond_block(2)), 0, st, T, ns, s->cap,
-                               cstart, spos, sdiam, bcount, (float *)s->d_bonds.p, wmax);
+        constexpr int B2 = bond_block(2), M2 = bond_list(2), B3 = bond_block(3), M3 = bond_list(3);
+        if (s->D == 2 && s->fp32)
+            hipLaunchKernelGGL((k_sweep_bonds<2, 2, float, B2, M2>), bg, dim3(B2), 0, st, T, ns, s->cap, cstart, spos,
+                               sdiam, bcount, (float *)s->d_bonds.p, wmax, rec);
+        else if (s->D == 2)
+            hipLaunchKernelGGL((k_sweep_bonds<2, 2, double, B2, M2>), bg, dim3(B2), 0, st, T, ns, s->cap, cstart, spos,
+                               sdiam, bcount, (double *)s->d_bonds.p, wmax, rec);
+        else if (s->fp32)
+            hipLaunchKernelGGL((k_sweep_bonds<3, 3, float, B3, M3>), bg, dim3(B3), 0, st, T, ns, s->cap, cstart, spos,
+                               sdiam, bcount, (float *)s->d_bonds.p, wmax, rec);
         else
-            hipLaunchKernelGGL((k_sweep_bonds<double, bond_block(2), bond_list(2)>), bg, dim3(bond_block(2)), 0, st, T, ns, s->cap,
-                               cstart, spos, sdiam, bcount, (double *)s->d_bonds.p, wmax);
+            hipLaunchKernelGGL((k_sweep_bonds<3, 3, double, B3, M3>), bg, dim3(B3), 0, st, T, ns, s->cap, cstart, spos,
+                               sdiam, bcount, (double *)s->d_bonds.p, wmax, rec);
         HIPCHK(hipGetLastError());
     }
     if (timed) HIPCHK(hipEventRecord(s->ev[1], st));
@@ -1870,14 +1887,15 @@ int sweep_launch(pgt_ik_sweep *s, bool timed)
     c.skin = s->kin ? skin : nullptr;
     c.cap = s->cap;
     c.bcount = s->vir ? bcount : nullptr;
+    c.rec = rec;
     c.bonds = s->vir ? (const void *)s->d_bonds.p : nullptr;
     c.wave_counts = (unsigned long long *)s->d_wcounts.p;
     c.flags = s->d_maxcount.p + 1;
     c.stream = st;
-    launch_sweep_gather(c, s->fp32, s->fields);
+    launch_sweep_gather(c, s->D, s->fp32, s->fields);
     HIPCHK(hipGetLastError());
     if (timed) HIPCHK(hipEventRecord(s->ev[2], st));
-    const int nwg = s->blocks[SW_GATHER], nwb = s->vir ? s->blocks[SW_BONDS] * (bond_block(2) / 64) : 0;
+    const int nwg = s->blocks[SW_GATHER], nwb = s->vir ? s->blocks[SW_BONDS] * (bond_block(s->D) / 64) : 0;
     const int nrb = std::max(1, std::min(256, (std::max(nwg, nwb) + 4095) / 4096));
     hipLaunchKernelGGL(k_reduce_stats, dim3(nrb), dim3(1024), 0, st, nwg, c.wave_counts, nwb, wmax, s->d_stats.p,
                        s->d_maxcount.p);
@@ -1913,6 +1931,7 @@ int pgt_ik_sweep_create(int nsys, const pgt_ik_sweep_system *systems, unsigned f
     HIPCHK(hipGetDevice(&dev));
     const bool vir = fields & PGT_FIELD_VIRIAL, kin = fields & PGT_FIELD_KINETIC;
     bool fp32 = false;
+    int D = 2;
     for (int k = 0; k < nsys; ++k) {
         const pgt_ik_sweep_system &y = systems[k];
         const pgt_ik_engine *e = y.engine;
@@ -1921,8 +1940,10 @@ int pgt_ik_sweep_create(int nsys, const pgt_ik_sweep_system *systems, unsigned f
         if (!e->box_set || !e->have_grid || e->G < 1)
             return fail(PGT_ERR_STATE, at + "set_box and a nonempty set_grid must precede the sweep");
         if (e->device != dev) return fail(PGT_ERR_ARG, at + "engine on another device");
-        if (e->prm.dim != 2 || !e->grid_z_flat)
-            return fail(PGT_ERR_UNSUPPORTED, at + "a sweep runs 2D systems (grid and positions with z = 0)");
+        if (e->prm.dim == 2 && !e->grid_z_flat)
+            return fail(PGT_ERR_UNSUPPORTED, at + "a sweep runs 2D systems with grid and positions at z = 0, or 3D ones");
+        if (k == 0) D = e->prm.dim;
+        else if (e->prm.dim != D) return fail(PGT_ERR_ARG, at + "every system of a sweep has the same dimension");
         if (e->prm.cg_kind != PGT_CG_OCTIC || e->prm.quad_mode != PGT_QUAD_FIXED || e->gl_s.size() != 5)
             return fail(PGT_ERR_UNSUPPORTED, at + "a sweep runs the Octic kernel with the fixed 5-node rule");
         const bool f32 = e->prm.precision == PGT_PREC_FP32;
@@ -1945,10 +1966,15 @@ int pgt_ik_sweep_create(int nsys, const pgt_ik_sweep_system *systems, unsigned f
     s->vir = vir;
     s->kin = kin;
     s->fp32 = fp32;
+    s->D = D;
     s->ns = nsys;
     s->R = engine_rule(systems[0].engine);
-    s->ebytes = fp32 ? sizeof(float) * BondLayout<2, 2, false, float>::ES
-                     : sizeof(double) * BondLayout<2, 2, false, double>::ES;
+    if (D == 2)
+        s->ebytes = fp32 ? sizeof(float) * BondLayout<2, 2, false, float>::ES
+                         : sizeof(double) * BondLayout<2, 2, false, double>::ES;
+    else
+        s->ebytes = fp32 ? sizeof(float) * BondLayout<3, 3, false, float>::ES
+                         : sizeof(double) * BondLayout<3, 3, false, double>::ES;
     s->tab.resize((size_t)nsys);
     int maxcell = 1, cap = 0;
     int64_t gtotal = 0;
@@ -1956,7 +1982,7 @@ int pgt_ik_sweep_create(int nsys, const pgt_ik_sweep_system *systems, unsigned f
         const pgt_ik_engine *e = systems[k].engine;
         SweepSys &S = s->tab[(size_t)k];
         S = SweepSys{};
-        S.g = make_geom(e, 2);
+        S.g = make_geom(e, D);
         S.P = make_pot(e->prm);
         S.cgnorm = cg_norm(e->prm.cg_kind, e->prm.cg_rcut);
         S.n = (int)systems[k].n;
@@ -1988,7 +2014,7 @@ int pgt_ik_sweep_create(int nsys, const pgt_ik_sweep_system *systems, unsigned f
         s->nh += (int64_t)S.nf * s->radix * S.nchunk;
         S.bpf[SW_SORT] = S.nchunk;
         S.bpf[SW_FINISH] = (S.n + 255) / 256;
-        S.bpf[SW_BONDS] = (S.n + bond_block(2) - 1) / bond_block(2);
+        S.bpf[SW_BONDS] = (S.n + bond_block(D) - 1) / bond_block(D);
         S.bpf[SW_GATHER] = (S.G + kTilePoints - 1) / kTilePoints;
         for (int st = 0; st < SW_NSTAGE; ++st) {
             const int64_t nb = (st == SW_SORT || st == SW_FINISH)
@@ -2042,7 +2068,7 @@ void pgt_ik_sweep_destroy(pgt_ik_sweep *s)
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     DBuf<unsigned char> *bufs[] = {&s->d_keys0, &s->d_vals0, &s->d_keys1, &s->d_vals1, &s->d_hist,
                                    &s->d_cstart, &s->d_spos, &s->d_sdiam, &s->d_bcount, &s->d_skin,
-                                   &s->d_bonds, &s->d_wcounts, &s->d_wmax};
+                                   &s->d_bonds, &s->d_wcounts, &s->d_wmax, &s->d_rec};
     for (auto *b : bufs) b->release();
     s->d_tab.release();
     s->d_grid.release();

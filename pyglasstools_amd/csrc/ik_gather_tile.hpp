@@ -514,23 +514,28 @@ __global__ __launch_bounds__(64, (TileWaves<D, ELA, KIN, ER>::value)) void k_gat
         (size_t)blockIdx.y * gridDim.x + blockIdx.x, xcd_tile(blockIdx.x, gridDim.x), overflow);
 }
 
-// The sweep's field gather (2D, virial and/or kinetic): each workgroup finds
-// its system, frame and tile in the sweep's table (ik_sweep.hpp).  Padding
-// workgroups store zero counts for their slot, so k_reduce_stats can read
-// every slot.  Three waves per SIMD: the system's geometry and pointers come
-// from the table and stay in registers (a kernel argument would be reloaded
-// instead), which at the single-system kernel's four waves spills 68 bytes a
-// lane in FP64; with the room of three there is no spill, and the eight-system
-// sweep's gather takes 6.27 ms instead of 6.54.
+// The sweep's field gather (2D or 3D, virial and/or kinetic): each
+// workgroup finds its system, frame and tile in the sweep's table
+// (ik_sweep.hpp).  Padding workgroups store zero counts for their slot, so
+// k_reduce_stats can read every slot.  Three waves per SIMD: the system's
+// geometry and pointers come from the table and stay in registers (a kernel
+// argument would be reloaded instead), which at the single-system kernel's
+// four waves spills 68 bytes a lane in the 2D FP64 form; with the room of
+// three there is no spill, and the eight-system 2D sweep's gather takes
+// 6.27 ms instead of 6.54.
 constexpr int kSweepGatherWaves = 3;
 
-template <bool VIR, bool KIN, typename ER>
+template <int D, int OD, bool VIR, bool KIN, typename ER>
 __global__ __launch_bounds__(64, kSweepGatherWaves) void k_sweep_gather(
     const SweepSys *__restrict__ T, int ns, GLRule R, const int *__restrict__ cstart,
     const double *__restrict__ spos, const double *__restrict__ skin, int cap, const int *__restrict__ bcount,
-    const ER *__restrict__ bonds, unsigned long long *__restrict__ wave_counts, int *__restrict__ overflow)
+    const float4 *__restrict__ rec, const ER *__restrict__ bonds, unsigned long long *__restrict__ wave_counts,
+    int *__restrict__ overflow)
 {
-    constexpr int ES = BondLayout<2, 2, false, ER>::ES;
+    constexpr int PS = (D == 2) ? 2 : 4;
+    constexpr int KN = OD * (OD + 1) / 2;
+    constexpr int ES = BondLayout<D, OD, false, ER>::ES;
+    constexpr bool RECS = VIR && D == 3;
     const SweepSys &S = T[sweep_system(T, ns, SW_GATHER, blockIdx.x)];
     int f, tile;
     if (!sweep_frame_tile(S, SW_GATHER, blockIdx.x, f, tile)) {
@@ -542,14 +547,14 @@ __global__ __launch_bounds__(64, kSweepGatherWaves) void k_sweep_gather(
     }
     const Geom &g = S.g;
     const int n = S.n, G = S.G;
-    const size_t fo = (size_t)S.p0 + (size_t)f * n, oo = (size_t)f * G * 4;
+    const size_t fo = (size_t)S.p0 + (size_t)f * n, oo = (size_t)f * G * OD * OD;
     GatherOut of{};
     of.vir = VIR ? S.vir + oo : nullptr;
     of.kin = KIN ? S.kin + oo : nullptr;
     of.wave_counts = wave_counts;
-    gather_tile_body<2, 2, 0, 5, VIR, false, KIN, ER>(
+    gather_tile_body<D, OD, 0, 5, VIR, false, KIN, ER>(
         g, n, G, S.grid, S.perm, R, nullptr, S.cgnorm, cstart + (size_t)S.c0 + (size_t)f * (g.ncell + 1),
-        spos + fo * 2, KIN ? skin + fo * 3 : nullptr, cap, VIR ? bcount + fo : nullptr, nullptr,
+        spos + fo * PS, KIN ? skin + fo * KN : nullptr, cap, VIR ? bcount + fo : nullptr, RECS ? rec + fo : nullptr,
         VIR ? bonds + fo * (size_t)cap * ES : nullptr, of, blockIdx.x, tile, overflow);
 }
 

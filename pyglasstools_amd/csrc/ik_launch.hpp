@@ -90,11 +90,13 @@ struct SweepGatherCtx {
     const double *spos, *skin;
     int cap;
     const int *bcount;
+    const float4 *rec;  // 3D: FP32 position and bond count per sorted particle
     const void *bonds;
     unsigned long long *wave_counts;
     int *flags;
     hipStream_t stream;
 };
-void launch_sweep_gather(const SweepGatherCtx &c, bool fp32, unsigned fields);  // ik_sweep_gather.hip
+// D: 2 (2 x 2 fields) or 3 (3 x 3)
+void launch_sweep_gather(const SweepGatherCtx &c, int D, bool fp32, unsigned fields);  // ik_sweep_gather.hip
 
 }  // namespace pgt

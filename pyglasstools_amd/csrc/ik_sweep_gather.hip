@@ -1,5 +1,5 @@
-// ik_sweep_gather.hip -- the sweep's field-gather instantiations (2D, Octic
-// 5-node rule, FP64 or FP32 bond entries; virial, kinetic or both).
+// ik_sweep_gather.hip -- the sweep's field-gather instantiations (2D or 3D,
+// Octic 5-node rule, FP64 or FP32 bond entries; virial, kinetic or both).
 #include "ik_gather_tile.hpp"
 #include "ik_launch.hpp"
 #include "pgt_ik.h"
@@ -8,29 +8,34 @@ namespace pgt {
 
 namespace {
 
-template <bool VIR, bool KIN, typename ER>
+template <int D, bool VIR, bool KIN, typename ER>
 void launch_t(const SweepGatherCtx &c)
 {
-    hipLaunchKernelGGL((k_sweep_gather<VIR, KIN, ER>), dim3((unsigned)c.blocks), dim3(kTileThreads), 0, c.stream,
-                       c.tab, c.ns, c.R, c.cstart, c.spos, c.skin, c.cap, c.bcount, (const ER *)c.bonds,
-                       c.wave_counts, c.flags);
+    hipLaunchKernelGGL((k_sweep_gather<D, D, VIR, KIN, ER>), dim3((unsigned)c.blocks), dim3(kTileThreads), 0,
+                       c.stream, c.tab, c.ns, c.R, c.cstart, c.spos, c.skin, c.cap, c.bcount, c.rec,
+                       (const ER *)c.bonds, c.wave_counts, c.flags);
 }
 
-template <typename ER>
+template <int D, typename ER>
 void launch_f(const SweepGatherCtx &c, unsigned fields)
 {
     const bool V = fields & PGT_FIELD_VIRIAL, K = fields & PGT_FIELD_KINETIC;
-    if (V && K) launch_t<true, true, ER>(c);
-    else if (V) launch_t<true, false, ER>(c);
-    else launch_t<false, true, ER>(c);
+    if (V && K) launch_t<D, true, true, ER>(c);
+    else if (V) launch_t<D, true, false, ER>(c);
+    else launch_t<D, false, true, ER>(c);
 }
 
 }  // namespace
 
-void launch_sweep_gather(const SweepGatherCtx &c, bool fp32, unsigned fields)
+void launch_sweep_gather(const SweepGatherCtx &c, int D, bool fp32, unsigned fields)
 {
-    if (fp32) launch_f<float>(c, fields);
-    else launch_f<double>(c, fields);
+    if (D == 2) {
+        if (fp32) launch_f<2, float>(c, fields);
+        else launch_f<2, double>(c, fields);
+    } else {
+        if (fp32) launch_f<3, float>(c, fields);
+        else launch_f<3, double>(c, fields);
+    }
 }
 
 }  // namespace pgt

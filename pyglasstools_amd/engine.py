@@ -399,12 +399,12 @@ class Engine(object):
 
 
 class Sweep(object):
-    """One pgt_ik_sweep: the frames of several 2D systems, one launch per
-    stage over all of them.  systems: dicts with the system's Engine (box,
-    grid and parameters set) under "engine", "nframes", "n", the device
-    inputs "pos", "diam" (and optionally "mass", "vel"; torch float64
+    """One pgt_ik_sweep: the frames of several systems (all 2D or all 3D),
+    one launch per stage over all of them.  systems: dicts with the system's
+    Engine (box, grid and parameters set) under "engine", "nframes", "n", the
+    device inputs "pos", "diam" (and optionally "mass", "vel"; torch float64
     tensors, frames back to back) and the outputs "virial" / "kinetic"
-    (nframes x G x 2 x 2) for the selected fields.  The sweep keeps the
+    (nframes x G x D x D) for the selected fields.  The sweep keeps the
     tensors alive and reads and writes them on every run."""
 
     def __init__(self, systems, fields=("virial", "kinetic"), stream_handle=None):
@@ -419,9 +419,7 @@ class Sweep(object):
             e, nf, n = y["engine"], int(y["nframes"]), int(y["n"])
             if nf < 1 or n < 1:
                 raise ValueError("system %d: frame and particle counts must be positive" % k)
-            m, fsz = nf * n, nf * e.G * 4
-            if e.dim != 2:
-                raise ValueError("system %d: a sweep runs 2D engines" % k)
+            m, fsz = nf * n, nf * e.G * e.dim * e.dim
             a = arr[k]
             a.engine = e.h.value if isinstance(e.h, ctypes.c_void_p) else e.h
             a.nframes, a.n = nf, n

@@ -1,6 +1,6 @@
-"""The sweep (pgt_ik_sweep_*): the frames of several 2D systems of uneven N,
-each with its own box, grid and search radius, computed with one launch per
-stage over all of them.  Each system's fields must equal, bit for bit, its
+"""The sweep (pgt_ik_sweep_*): the frames of several systems of uneven N (all
+2D or all 3D), each with its own box, grid and search radius, computed with
+one launch per stage over all of them.  Each system's fields must equal, bit for bit, its
 own engine's compute_frames with the tile gather (the kernels' bodies are
 the same; only the workgroup-to-frame mapping differs), and the FP64 fields
 the oracle at the drop-in tolerance; replays of a captured graph of the
@@ -216,30 +216,90 @@ def test_sweep_kinetic_only(gpu):
     assert sw.stats()["n_contrib"] == 0
 
 
+def _systems_3d(spec, precision="fp64", seed0=7500):
+    """spec: (N, frames, grid side) per 3D system."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    out = []
+    for k, (N, FR, NG) in enumerate(spec):
+        frames = [workload.make_system(N, 3, seed0 + 100 * k + f) for f in range(FR)]
+        L = frames[0]["L"]
+        d0 = max(float(f["diam"].max()) for f in frames)
+        max_rcut = max(1.25 * d0, d0)
+        grid = workload.cartesian_grid_3d(L[0], NG)
+        e = engine.Engine(dim=3, max_rcut=max_rcut, precision=precision)
+        e.set_box(L, [1, 1, 1])
+        e.set_grid(grid)
+        e.set_gather_mode("tile")
+        G = len(grid)
+        y = dict(engine=e, nframes=FR, n=N, frames=frames, grid=grid, max_rcut=max_rcut,
+                 pos=_stack(frames, "pos", dev), diam=_stack(frames, "diam", dev),
+                 mass=_stack(frames, "mass", dev), vel=_stack(frames, "vel", dev))
+        for name in ("virial", "kinetic", "ref_virial", "ref_kinetic"):
+            y[name] = torch.full((FR, G, 3, 3), float("nan"), dtype=torch.float64, device=dev)
+        out.append(y)
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_sweep_3d_matches_engines_and_oracle(gpu, precision):
+    """Three 3D systems (N 800 to 3,000; 1 or 2 frames; 6^3 to 8^3 grids): the
+    sweep against each engine bit for bit (FP32 records, band pre-tests and
+    all), the FP64 fields against the oracle, the FP32 ones within 1e-5 of
+    each component's largest |T|."""
+    fields = ("virial", "kinetic")
+    systems = _systems_3d(((800, 2, 6), (3000, 1, 8), (1700, 2, 7)), precision)
+    nc = _engine_reference(systems, fields)
+    sw = engine.Sweep(systems, fields=fields)
+    print("3d sweep %s: run" % precision, flush=True)
+    sw.run()
+    _assert_same(systems, fields, "3d sweep " + precision)
+    assert sw.stats()["n_contrib"] == nc
+    for k, y in enumerate(systems):
+        for f in range(y["nframes"]):
+            ref = oracle.ik_compute(y["frames"][f], y["grid"], fields=fields, max_rcut=y["max_rcut"], with_abs=True)
+            gv = y["virial"][f].cpu().numpy()
+            gk = y["kinetic"][f].cpu().numpy()
+            if precision == "fp64":
+                err = scaled_error(gv, ref["virial"], ref["virial_abs"], floor=1e-4).max()
+                assert err <= TOL, (k, f, err)
+            else:
+                for a in range(3):
+                    for b in range(a, 3):
+                        r = ref["virial"][:, b, a]
+                        assert np.abs(gv[:, b, a] - r).max() <= 1e-5 * np.abs(r).max(), (k, f, a, b)
+            assert np.abs(gk - ref["kinetic"]).max() <= 1e-12 * np.abs(ref["kinetic"]).max(), (k, f)
+
+
 def test_sweep_refuses_unsupported_settings(gpu):
-    """3D engines, other kernels or orders, the elastic field and mixed
-    precisions are refused at create; enqueue before a run is a state error;
-    particle counts past the tile gather's 2^26 are refused before any
-    pointer is read."""
+    """Mixed dimensions or precisions, a 2D grid off z = 0, other kernels or
+    orders and the elastic field are refused at create; enqueue before a run
+    is a state error; particle counts past the tile gather's 2^26 are refused
+    before any pointer is read."""
     import torch
 
     good = _systems(SPEC[:1])
     y = good[0]
-    s3 = workload.make_system(300, 3, seed=3)
-    e3 = engine.Engine(dim=3, diameters=s3["diam"])
-    e3.set_box(s3["L"], [1, 1, 1])
-    e3.set_grid(workload.cartesian_grid_3d(s3["L"][0], 4))
-    cases = [(e3, -5)]
+    y3 = _systems_3d(((300, 1, 4),))[0]
+    with pytest.raises(engine.PGTError) as ex:
+        engine.Sweep([y, y3], fields=("virial",))
+    assert ex.value.code == -1
+    e_off = engine.Engine(dim=2, max_rcut=y["max_rcut"])
+    e_off.set_box(y["frames"][0]["L"], [1, 1, 0])
+    g_off = y["grid"].copy()
+    g_off[:, 2] = 0.5
+    e_off.set_grid(g_off)
+    cases = [(e_off, -5)]
     for kw in (dict(cg="rect"), dict(order=6)):
         e = engine.Engine(dim=2, max_rcut=y["max_rcut"], **kw)
         e.set_box(y["frames"][0]["L"], [1, 1, 0])
         e.set_grid(y["grid"])
         cases.append((e, -5))
     for e, code in cases:
-        with pytest.raises((engine.PGTError, ValueError)) as ex:
+        with pytest.raises(engine.PGTError) as ex:
             engine.Sweep([dict(y, engine=e)], fields=("virial",))
-        if isinstance(ex.value, engine.PGTError):
-            assert ex.value.code == code
+        assert ex.value.code == code
     with pytest.raises(engine.PGTError) as ex:
         engine.Sweep(good, fields=("virial", "elastic"))
     assert ex.value.code == -5
