@@ -288,6 +288,9 @@ struct pgt_ik_engine {
     // capacity that entry point uses
     unsigned char *scratch = nullptr;
     int scratch_cap = 0;
+    // pinned host copy of a job's counters (flags, then the two totals), read
+    // back with one synchronisation per job
+    unsigned long long *h_counts = nullptr;
     pgt_ik_stats stats{};
 };
 
@@ -814,9 +817,13 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
                            e->ev[3 * b + 1], e->ev[3 * b + 2]);
             if (rc) return rc;
         }
-        int flags[2] = {0, 0};
-        HIPCHK(hipMemcpyAsync(flags, e->d_maxcount.p, sizeof(flags), hipMemcpyDeviceToHost, e->stream));
+        // flags and totals into pinned memory, one synchronisation
+        int *hf = reinterpret_cast<int *>(e->h_counts);
+        HIPCHK(hipMemcpyAsync(hf, e->d_maxcount.p, 2 * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipMemcpyAsync(e->h_counts + 1, e->d_stats.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              e->stream));
         HIPCHK(hipStreamSynchronize(e->stream));
+        const int flags[2] = {hf[0], hf[1]};
         mx = flags[0];
         const bool grow = j.pairs && mx > e->cap;
         if (grow && mx > 65535) return fail(PGT_ERR_RANGE, "a particle has more than 65535 interacting neighbours");
@@ -854,8 +861,7 @@ int run_job(pgt_ik_engine *e, const FrameSet &fs, unsigned fields)
         if (grow) e->cap = ((mx + 7) / 8) * 8;
         if (attempt == 3) return fail(PGT_ERR_HIP, "bond list capacity did not settle");
     }
-    unsigned long long h[2] = {0, 0};
-    HIPCHK(hipMemcpy(h, e->d_stats.p, sizeof(h), hipMemcpyDeviceToHost));
+    const unsigned long long h[2] = {e->h_counts[1], e->h_counts[2]};
     double g_ms = 0.0;
     for (int b = 0; b < nb; ++b) {
         float t = 0.0f;
@@ -1030,6 +1036,12 @@ int pgt_ik_create(const pgt_ik_params *params, pgt_ik_engine **out)
         return fail(PGT_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
     e->stream = e->own_stream;
+    he = hipHostMalloc((void **)&e->h_counts, 4 * sizeof(unsigned long long), hipHostMallocDefault);
+    if (he != hipSuccess) {
+        e->h_counts = nullptr;
+        pgt_ik_destroy(e);
+        return fail(PGT_ERR_HIP, std::string("hipHostMalloc: ") + hipGetErrorString(he));
+    }
     if (gl_rule(params->quad_mode == PGT_QUAD_FIXED ? params->quad_order : 5, e->gl_s, e->gl_w) != 0) {
         pgt_ik_destroy(e);
         return fail(PGT_ERR_ARG, "bad quadrature order");
@@ -1071,6 +1083,7 @@ void pgt_ik_destroy(pgt_ik_engine *e)
     e->d_pn.release();
     e->d_stats.release();
     for (hipEvent_t v : e->ev) (void)hipEventDestroy(v);
+    if (e->h_counts) (void)hipHostFree(e->h_counts);
     if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
     delete e;
 }
