@@ -1761,7 +1761,7 @@ int pgt_ik_compute_surrogate(pgt_ik_engine *e, const float *d_params, double bou
 }  // extern "C"
 
 // ==========================================================================
-// sweep: the frames of several 2D systems, one launch per stage
+// sweep: the frames of several systems, all 2D or all 3D, one launch per stage
 // ==========================================================================
 struct pgt_ik_sweep {
     int device = 0;

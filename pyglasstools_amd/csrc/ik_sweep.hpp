@@ -1,7 +1,7 @@
-// ik_sweep.hpp -- the sweep: the frames of several 2D systems (uneven
-// particle counts, each with its own box, grid, cell geometry and pair
-// potential) computed with one launch per stage over all of them, instead of
-// one launch per stage and system.  The kernels' bodies are those of the
+// ik_sweep.hpp -- the sweep: the frames of several systems, all 2D or all 3D
+// (uneven particle counts, each with its own box, grid, cell geometry and
+// pair potential), computed with one launch per stage over all of them,
+// instead of one launch per stage and system.  The kernels' bodies are those of the
 // single-system launches; only the way a workgroup finds its system, frame
 // and block differs.
 #pragma once
