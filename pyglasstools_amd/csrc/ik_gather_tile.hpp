@@ -11,7 +11,10 @@
 //    tests them exactly (FP64 minimum image, |a|^2 < r_cg^2; in 3D with
 //    bonds an FP32 test on the bond kernel's records first, with the FP64
 //    test for the band where FP32 rounding could decide).  The items found
-//    go to a list shared by the wavefront (ballot, prefix count).
+//    go to a list shared by the wavefront (ballot, prefix count).  A 2D
+//    scan reads candidate positions only; the bond counts of the items found
+//    are loaded before the sort, a batch per lane (in 3D the record holds
+//    the count).
 // 2. When the list could overflow, and at the end: the list is sorted by
 //    the particle's bond count, largest first (stable counting sort), and
 //    lane l takes items l, l + 64, ...: the 64 items of a step have nearly
@@ -126,6 +129,7 @@ __device__ __forceinline__ void gather_tile_body(const Geom &g, int n, int G, co
 
     unsigned long long nball = 0;  // items listed (the same in every lane)
     unsigned long long ncontrib = 0;  // bond counts of the items, per lane
+    bool over = false;  // an item's bond count exceeds MAXNB
     const double s2 = 2.0 * g.inv_rc;
 
     // ---- sort the first ni items of the list by bond count and integrate --
@@ -136,10 +140,36 @@ __device__ __forceinline__ void gather_tile_body(const Geom &g, int n, int G, co
         s_bucket[l] = 0;
         __syncthreads();
         nball += (unsigned long long)ni;
-        for (int k = l; k < ni; k += 64) {
-            const int nbk = s_inb[k];
-            ncontrib += (unsigned long long)nbk;
-            atomicAdd(&s_bucket[63 - min(nbk, 63)], 1);
+        if constexpr (PAIRS && !RECS) {
+            // the items' bond counts, loaded here U at a time rather than
+            // with every scanned candidate (2D scans read positions only)
+            constexpr int U = 8;
+            for (int k0 = l; k0 < ni; k0 += 64 * U) {
+                int nbv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int k = k0 + 64 * u;
+                    nbv[u] = k < ni ? bc[s_item[k] & PMASK] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int k = k0 + 64 * u;
+                    if (k < ni) {
+                        const int nbc = min(nbv[u], cap);
+                        if (nbc > CP::MAXNB) over = true;
+                        const int nbk = min(nbc, CP::MAXNB);
+                        s_inb[k] = (unsigned char)nbk;
+                        ncontrib += (unsigned long long)nbk;
+                        atomicAdd(&s_bucket[63 - min(nbk, 63)], 1);
+                    }
+                }
+            }
+        } else {
+            for (int k = l; k < ni; k += 64) {
+                const int nbk = s_inb[k];
+                ncontrib += (unsigned long long)nbk;
+                atomicAdd(&s_bucket[63 - min(nbk, 63)], 1);
+            }
         }
         __syncthreads();
         {
@@ -340,7 +370,7 @@ __device__ __forceinline__ void gather_tile_body(const Geom &g, int n, int G, co
     // candidates c0 .. c0 + NB - 1 of a row (clamped into it; a lane with an
     // empty row reads particle 0, and every lane has one when any reads)
     struct Cand {
-        int p[NB], nb[NB];
+        int p[NB], nb[RECS ? NB : 1];
         double q[RECS ? 1 : NB][3];
         float r[RECS ? NB : 1][3];
     };
@@ -359,7 +389,6 @@ __device__ __forceinline__ void gather_tile_body(const Geom &g, int n, int G, co
                 C.q[u][0] = sp[(size_t)C.p[u] * PS];
                 C.q[u][1] = sp[(size_t)C.p[u] * PS + 1];
                 C.q[u][2] = (D == 3) ? sp[(size_t)C.p[u] * PS + 2] : 0.0;
-                C.nb[u] = PAIRS ? bc[C.p[u]] : 0;
             }
         }
     };
@@ -373,7 +402,6 @@ __device__ __forceinline__ void gather_tile_body(const Geom &g, int n, int G, co
         const double a2 = (D == 3) ? minimg(q[2] - xg[2], g.L[2], g.pinvL[2]) : 0.0;
         return a0 * a0 + a1 * a1 + a2 * a2 < g.rc2;
     };
-    bool over = false;  // an item's bond count exceeds MAXNB
     // the scan visits rows r = 0 .. nrows - 1, each in steps of NB candidates
     // per lane (one step for a row that is empty in every lane)
     auto test_append = [&](const Cand &C, int c0, int len, int &ni) {
@@ -404,10 +432,14 @@ __device__ __forceinline__ void gather_tile_body(const Geom &g, int n, int G, co
             if (in) {
                 const int k = ni + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32),
                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
-                const int nbc = PAIRS ? min(C.nb[u], cap) : 0;
-                if (nbc > CP::MAXNB) over = true;
                 s_item[k] = (unsigned)C.p[u] | ((unsigned)l << PBITS);
-                s_inb[k] = (unsigned char)min(nbc, CP::MAXNB);
+                if constexpr (RECS) {
+                    const int nbc = min(C.nb[u], cap);
+                    if (nbc > CP::MAXNB) over = true;
+                    s_inb[k] = (unsigned char)min(nbc, CP::MAXNB);
+                } else if constexpr (!PAIRS) {
+                    s_inb[k] = 0;
+                }
             }
             ni += __popcll(bm);
         }
