@@ -212,22 +212,24 @@ __global__ __launch_bounds__(BLK) void k_gather(Geom g, int n, int G, const doub
                             v[4] = fma(cb0, e[2], v[4]);
                             v[5] = fma(cb1, e[2], v[5]);
                         }
+                    } else if constexpr (OD == 2) {
+                        // FP32 entries, 2D: products in FP32, sums in FP64
+                        // (as the tile gather)
+                        v[0] += (double)(cb0 * e[0]);
+                        v[1] += (double)(cb1 * e[1]);
+                        v[2] += (double)(cb0 * e[1]);
                     } else {
-                        // FP32 entries: c' w and b' widened, the products
+                        // FP32 entries, 3D: c' w and b' widened, the products
                         // and sums in FP64
                         const double cd = (double)c, d0 = (double)e[0], d1 = (double)e[1];
-                        const double q0 = cd * d0, q1 = cd * d1;
+                        const double d2 = (double)e[2];
+                        const double q0 = cd * d0, q1 = cd * d1, q2 = cd * d2;
                         v[0] = fma(q0, d0, v[0]);
                         v[1] = fma(q1, d1, v[1]);
-                        if (OD == 2) {
-                            v[2] = fma(q0, d1, v[2]);
-                        } else {
-                            const double d2 = (double)e[2], q2 = cd * d2;
-                            v[2] = fma(q2, d2, v[2]);
-                            v[3] = fma(q0, d1, v[3]);
-                            v[4] = fma(q0, d2, v[4]);
-                            v[5] = fma(q1, d2, v[5]);
-                        }
+                        v[2] = fma(q2, d2, v[2]);
+                        v[3] = fma(q0, d1, v[3]);
+                        v[4] = fma(q0, d2, v[4]);
+                        v[5] = fma(q1, d2, v[5]);
                     }
                 }
                 if (ELA) {

@@ -263,22 +263,25 @@ __device__ __forceinline__ void gather_tile_body(const Geom &g, int n, int G, co
                                     acc[4] = fma(cb0, e[2], acc[4]);
                                     acc[5] = fma(cb1, e[2], acc[5]);
                                 }
+                            } else if constexpr (OD == 2) {
+                                // FP32 entries, 2D: products in FP32, sums
+                                // in FP64 (the registers of the 3D form
+                                // below would spill here)
+                                acc[0] += (double)(cb0 * e[0]);
+                                acc[1] += (double)(cb1 * e[1]);
+                                acc[2] += (double)(cb0 * e[1]);
                             } else {
-                                // FP32 entries: c' w and b' widened, the
+                                // FP32 entries, 3D: c' w and b' widened, the
                                 // products and sums in FP64
                                 const double cd = (double)c, d0 = (double)e[0], d1 = (double)e[1];
-                                const double q0 = cd * d0, q1 = cd * d1;
+                                const double d2 = (double)e[2];
+                                const double q0 = cd * d0, q1 = cd * d1, q2 = cd * d2;
                                 acc[0] = fma(q0, d0, acc[0]);
                                 acc[1] = fma(q1, d1, acc[1]);
-                                if constexpr (OD == 2) {
-                                    acc[2] = fma(q0, d1, acc[2]);
-                                } else {
-                                    const double d2 = (double)e[2], q2 = cd * d2;
-                                    acc[2] = fma(q2, d2, acc[2]);
-                                    acc[3] = fma(q0, d1, acc[3]);
-                                    acc[4] = fma(q0, d2, acc[4]);
-                                    acc[5] = fma(q1, d2, acc[5]);
-                                }
+                                acc[2] = fma(q2, d2, acc[2]);
+                                acc[3] = fma(q0, d1, acc[3]);
+                                acc[4] = fma(q0, d2, acc[4]);
+                                acc[5] = fma(q1, d2, acc[5]);
                             }
                         }
                         if constexpr (ELA) {
